@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: SSB Q3.3-shaped LIP star join on one MI355X.
+
+Workload: 120,000,000-row fact table (lo_custkey, lo_suppkey, lo_orderdate,
+lo_partkey, lo_revenue), customer 600,000 / supplier 40,000 / date 2,556 rows,
+s_city, c_city in {231, 235}, 1992 <= d_year <= 1997, GROUP BY (c_city, s_city,
+d_year) with int64 SUM(lo_revenue).  Synthetic, seeded, generated on the device.
+
+A step is one whole query, the reference's runQuery window: table fills,
+the three dimension builds, the fused probe and the group finalize (the
+results are copied back and checked each step).  Timed steps use the
+adaptive-LIP plan; every one of the reference's 13 plans (6 join orders x
+{naive, LIP} + best-bloom) is also timed and reported under "plans".
+
+Roofline: the fused probe kernel is the dominant kernel.  Its algorithmic
+bytes per launch are 128 B x the 128-byte lines of fact columns that hold at
+least one row still alive when the column is read (exact counts reported by
+the kernel, equal to the oracle's in the parity tests), using the minimum
+over the 6 static LIP orders; achieved = bytes / average probe-kernel time
+(HIP events on the kernel's stream); peak = 8 TB/s HBM3E.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "crystal-opt_amd"))
+
+METRIC = "fact rows/s + %HBM peak, LIP vs naive star join; batched p50/p99 latency"
+HBM_PEAK = 8.0e12
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n_fact, seed, threads):
+    """Oracle (C restatement of the reference semantics) on the host cores:
+    the LIP plan in join order S->C->D, whole 120M-row input, median of 5
+    after one warm-up."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from q33_helpers import oracle_q33
+    from lipjoin import workloads as W
+    host_dims = W.q33_host_dims(seed)
+    host_fact = W.q33_host_fact(seed, n_fact, ["lo_suppkey", "lo_custkey", "lo_orderdate", "lo_revenue"])
+    ts = []
+    res = None
+    for i in range(6):
+        t0 = time.perf_counter()
+        res = oracle_q33(host_dims, host_fact, W.Q33_ORDERS[0], "lip", threads=threads)
+        dt = time.perf_counter() - t0
+        if i > 0:
+            ts.append(dt)
+    t = statistics.median(ts)
+    return {"value": n_fact / t, "unit": "fact rows/s", "cores": threads, "kind": "port",
+            "sample": f"full {n_fact}-row Q3.3 input, LIP order S->C->D (build+probe), median of 5 "
+                      f"after 1 warm-up, {threads} threads",
+            "seconds": t}, res[0]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--fact-rows", type=int, default=120_000_000)
+    ap.add_argument("--plan-reps", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plans", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the star join shards by fact "
+              "rows with replicated dimensions, see DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import lipjoin as L
+    from lipjoin import workloads as W
+
+    n = args.fact_rows
+    seed = W.seed_of(3)
+    ctx = L.Context(0)
+    log(f"generating {n} fact rows on the device")
+    cols = W.q33_device_columns(ctx, seed, n)
+    torch.cuda.synchronize()
+
+    def run(method, order, bloom_order=None):
+        return ctx.star_join(W.q33_star_desc(cols, n, method, order, bloom_order))
+
+    # ---- per-plan comparison (naive vs LIP in every join order) -----------------
+    plans = {}
+    ref_groups = None
+    lip_bytes = []
+    for name, method, order, bloom_order in W.q33_plans():
+        if args.no_plans and name != "lip-adaptive":
+            continue
+        res = run(method, order, bloom_order)
+        if ref_groups is None:
+            ref_groups = res["groups"]
+        assert res["groups"] == ref_groups, f"plan {name} disagrees"
+        ts = []
+        tp = []
+        for _ in range(args.plan_reps):
+            r = run(method, order, bloom_order)
+            ts.append(r["time_query_ms"])
+            tp.append(r["time_probe_ms"])
+        st = res["stats"]
+        lines = sum(st["lines_in"][:4]) + st["sample_lines"]
+        bytes_alg = 128 * lines
+        if method == "lip":
+            if bloom_order is None:
+                lip_bytes.append(bytes_alg)
+        plans[name] = {"query_ms": statistics.median(ts), "probe_ms": statistics.median(tp),
+                       "rows_per_s": n / (statistics.median(ts) * 1e-3), "bytes_alg": bytes_alg,
+                       "probe_hbm_frac": bytes_alg / (statistics.median(tp) * 1e-3) / HBM_PEAK,
+                       "order_used": st["order_used"][:3]}
+        log(f"plan {name:18s} query {plans[name]['query_ms']:.3f} ms probe {plans[name]['probe_ms']:.3f} ms "
+            f"bytes {bytes_alg / 1e6:.1f} MB")
+    speedup = {}
+    for i in range(6):
+        if f"{i}-naive" in plans and f"{i}-lip" in plans:
+            speedup[W.ORDER_NAMES[i]] = plans[f"{i}-naive"]["query_ms"] / plans[f"{i}-lip"]["query_ms"]
+
+    # ---- timed steps: adaptive LIP, whole query per step ----------------------------
+    for _ in range(args.warmup):
+        run("lip-adaptive", W.Q33_ORDERS[0])
+    torch.cuda.synchronize()
+    probe_ms = []
+    query_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = run("lip-adaptive", W.Q33_ORDERS[0])
+        probe_ms.append(r["time_probe_ms"])
+        query_ms.append(r["time_query_ms"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if ref_groups is not None:
+        assert r["groups"] == ref_groups
+    ms_per_step = wall / args.steps * 1e3
+    value = n * args.steps / wall
+
+    probe_avg_s = statistics.fmean(probe_ms) * 1e-3
+    if lip_bytes:
+        bytes_alg = min(lip_bytes)
+    else:
+        st = r["stats"]
+        bytes_alg = 128 * (sum(st["lines_in"][:4]) + st["sample_lines"])
+    achieved = bytes_alg / probe_avg_s
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None,
+                "kernel": "lj::probe_kernel (fused LIP scan)", "bytes_alg_per_launch": bytes_alg,
+                "probe_ms_avg": probe_avg_s * 1e3}
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "fact rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32 keys, int64 sums", "data": "synthetic (seeded, generated on device)",
+        "config": {"workload": "SSB Q3.3 star join, LIP with device-chosen filter order",
+                   "fact_rows": n, "dims": {"customer": 600000, "supplier": 40000, "date": 2556},
+                   "groups": len(ref_groups) if ref_groups is not None else None, "parallelism": "single GPU"},
+        "query_ms_median": statistics.median(query_ms),
+        "whole_query_hbm_frac": bytes_alg / (statistics.median(query_ms) * 1e-3) / HBM_PEAK,
+        "roofline": roofline,
+        "lip_vs_naive_speedup": speedup,
+        "plans": plans,
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads")
+        cb, cpu_groups = cpu_baseline(n, seed, threads)
+        gpu_groups = [(g, tuple(p), s, c) for (g, p, s, c, _) in r["groups"]]
+        cb["matches_gpu"] = cpu_groups == gpu_groups
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

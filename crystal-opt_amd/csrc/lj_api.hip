@@ -1,0 +1,862 @@
+// lj_api.hip -- host side of the C-ABI in include/lipjoin.h.
+//
+// Orchestration mirrors the reference drivers' runQuery / hashJoin
+// (src/ssb-jo/q33/0-lip.cu:205-302, src/ops/bloom_join.cu:121-156): allocate and
+// clear the tables, build every dimension, run the fused probe, collect the
+// groups.  Differences by design: tables come from a per-context caching
+// allocator (released back to it, never leaked), fills are kernels (so a query
+// can be graph-captured), all dimensions are built by one launch, the group
+// arrays are kept zero by the finalize kernel instead of a 7 MB memset per
+// query, and every failure is a status code.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "lipjoin.h"
+#include "lj_gen.hpp"
+#include "lj_internal.hpp"
+
+using namespace lj;
+
+namespace lj {
+int64_t build_tile_rows();
+}
+
+// ------------------------------------------------------------------ context --
+struct lj_dim {
+  lj_dim_desc desc;
+  int32_t* ht = nullptr;
+  int64_t ht_words = 0;
+  uint32_t* bloom = nullptr;
+  int64_t bloom_words = 0;
+  unsigned long long* counters = nullptr;  // device [0] selected, [1] oob, [2] popcount scratch
+  BloomHash bh{};
+  FastMod ht_len{};
+  int64_t n_selected = -1;
+  int64_t n_oob = -1;
+  float time_build_ms = 0.f;
+};
+
+struct lj_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint64_t budget = 0;
+  uint64_t in_use = 0;  // bytes handed out
+  uint64_t cached = 0;  // bytes parked in the free lists
+  std::string err;
+  std::multimap<size_t, void*> free_blocks;
+  std::unordered_map<void*, size_t> live;
+  int n_cu = 0;
+  int max_lds = 65536;
+  hipEvent_t ev[6]{};
+  // group state (kept zero between queries by the finalize kernel)
+  int64_t g_cap = 0;
+  unsigned long long* gsum = nullptr;
+  unsigned long long* gcount = nullptr;
+  unsigned long long* gprod = nullptr;
+  int32_t* gpay = nullptr;
+  uint32_t* touched = nullptr;
+  uint32_t touched_cap = 0;
+  uint32_t* n_touched = nullptr;
+  GroupOut* out = nullptr;
+  long long* out_hdr = nullptr;
+  long long* stats = nullptr;
+  int stats_cap = 0;
+  void* host_buf = nullptr;  // pinned: header + first kHostGroups groups
+  // tuning (environment overrides for sweeps: LJ_PROBE_WG, LJ_PROBE_VEC, LJ_LDS_POLICY)
+  int probe_wg = 0;
+  int probe_vec = 0;
+  int lds_policy = -1;
+};
+
+static const int kHostGroups = 4096;
+static thread_local std::string g_create_err;
+
+static lj_status fail(lj_ctx* c, lj_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+static lj_status fail(lj_ctx* c, lj_status s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  else g_create_err = buf;
+  return s;
+}
+
+#define HIP_TRY(c, expr)                                                                           \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return fail(c, LJ_EHIP, "%s: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+// ---------------------------------------------------------- caching allocator --
+static size_t round_block(size_t bytes) {
+  size_t b = 256;
+  while (b < bytes) b <<= 1;
+  return b;
+}
+
+static lj_status dev_alloc(lj_ctx* c, size_t bytes, void** out) {
+  const size_t sz = round_block(bytes ? bytes : 1);
+  auto it = c->free_blocks.find(sz);
+  if (it != c->free_blocks.end()) {
+    *out = it->second;
+    c->free_blocks.erase(it);
+    c->cached -= sz;
+    c->live[*out] = sz;
+    c->in_use += sz;
+    return LJ_OK;
+  }
+  if (c->budget && c->in_use + sz > c->budget)
+    return fail(c, LJ_EBUDGET, "allocation of %zu bytes exceeds the HBM budget (%llu in use of %llu)", sz,
+                (unsigned long long)c->in_use, (unsigned long long)c->budget);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, sz);
+  if (e != hipSuccess) {
+    // give the cache back and retry once
+    (void)hipGetLastError();
+    for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
+    c->free_blocks.clear();
+    c->cached = 0;
+    e = hipMalloc(&p, sz);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, LJ_ENOMEM, "hipMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
+    }
+  }
+  c->live[p] = sz;
+  c->in_use += sz;
+  *out = p;
+  return LJ_OK;
+}
+
+static void dev_release(lj_ctx* c, void* p) {
+  if (!p) return;
+  auto it = c->live.find(p);
+  if (it == c->live.end()) return;
+  c->free_blocks.emplace(it->second, p);
+  c->cached += it->second;
+  c->in_use -= it->second;
+  c->live.erase(it);
+}
+
+template <typename T>
+static lj_status dev_alloc_t(lj_ctx* c, int64_t count, T** out) {
+  void* p = nullptr;
+  lj_status s = dev_alloc(c, (size_t)count * sizeof(T), &p);
+  *out = static_cast<T*>(p);
+  return s;
+}
+
+// -------------------------------------------------------------- ABI basics --
+extern "C" int32_t lj_abi_version(void) { return LJ_ABI_VERSION; }
+
+extern "C" int64_t lj_abi_struct_size(int32_t which) {
+  switch (which) {
+    case 0: return sizeof(lj_dim_desc);
+    case 1: return sizeof(lj_dim_info);
+    case 2: return sizeof(lj_probe_stats);
+    case 3: return sizeof(lj_join_desc);
+    case 4: return sizeof(lj_join_result);
+    case 5: return sizeof(lj_star_desc);
+    case 6: return sizeof(lj_group);
+    case 7: return sizeof(lj_star_result);
+  }
+  return -1;
+}
+
+extern "C" const char* lj_status_str(lj_status s) {
+  switch (s) {
+    case LJ_OK: return "ok";
+    case LJ_EINVAL: return "invalid argument";
+    case LJ_ENOMEM: return "out of device memory";
+    case LJ_EBUDGET: return "HBM budget exceeded";
+    case LJ_EHIP: return "HIP runtime error";
+    case LJ_EUNSUPPORTED: return "unsupported";
+    case LJ_EDOMAIN: return "key outside the hash domain";
+    case LJ_ETIMEOUT: return "timeout";
+    case LJ_ESTATE: return "invalid state";
+  }
+  return "unknown status";
+}
+
+extern "C" const char* lj_last_error(const lj_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+static int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return v && *v ? atoi(v) : dflt;
+}
+
+extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* hip_stream, lj_ctx** out) {
+  if (!out) return fail(nullptr, LJ_EINVAL, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return fail(nullptr, LJ_EHIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= ndev) return fail(nullptr, LJ_EINVAL, "device %d out of range [0,%d)", device, ndev);
+  lj_ctx* c = new lj_ctx();
+  c->device = device;
+  c->budget = hbm_budget_bytes;
+  auto bail = [&](lj_status s) {
+    g_create_err = c->err;
+    delete c;
+    return s;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(fail(c, LJ_EHIP, "hipSetDevice failed"));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail(c, LJ_EHIP, "hipGetDeviceProperties failed"));
+  c->n_cu = prop.multiProcessorCount;
+  c->max_lds = (int)prop.sharedMemPerBlock;
+  if (hip_stream) {
+    c->stream = static_cast<hipStream_t>(hip_stream);
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+      return bail(fail(c, LJ_EHIP, "hipStreamCreate failed"));
+    c->own_stream = true;
+  }
+  for (auto& ev : c->ev)
+    if (hipEventCreate(&ev) != hipSuccess) return bail(fail(c, LJ_EHIP, "hipEventCreate failed"));
+  if (hipHostMalloc(&c->host_buf, sizeof(long long) * (1 + kStats) + sizeof(GroupOut) * kHostGroups) != hipSuccess)
+    return bail(fail(c, LJ_EHIP, "hipHostMalloc failed"));
+  c->probe_wg = env_int("LJ_PROBE_WG", 0);
+  c->probe_vec = env_int("LJ_PROBE_VEC", 0);
+  c->lds_policy = env_int("LJ_LDS_POLICY", -1);
+  *out = c;
+  return LJ_OK;
+}
+
+extern "C" void lj_destroy(lj_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->live) (void)hipFree(kv.first);
+  for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
+  for (auto& ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->host_buf) (void)hipHostFree(c->host_buf);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" lj_status lj_sync(lj_ctx* c) {
+  if (!c) return LJ_EINVAL;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+extern "C" uint64_t lj_bytes_in_use(const lj_ctx* c) { return c ? c->in_use : 0; }
+
+// ------------------------------------------------------------- dimensions --
+static int ilog2_u64(uint64_t v) {
+  int r = -1;
+  while (v) {
+    v >>= 1;
+    r++;
+  }
+  return r;
+}
+
+static lj_status check_dim_desc(lj_ctx* c, const lj_dim_desc* d) {
+  if (!d) return fail(c, LJ_EINVAL, "dimension descriptor is NULL");
+  if (d->n_rows < 0 || d->n_rows >= (1ll << 31)) return fail(c, LJ_EINVAL, "dimension n_rows %lld out of range", (long long)d->n_rows);
+  if (d->n_rows > 0 && !d->keys) return fail(c, LJ_EINVAL, "dimension keys is NULL");
+  if (d->pred.n_steps < 0 || d->pred.n_steps > LJ_MAX_PRED_STEPS)
+    return fail(c, LJ_EINVAL, "predicate has %d steps (max %d)", d->pred.n_steps, LJ_MAX_PRED_STEPS);
+  for (int i = 0; i < d->pred.n_steps; i++) {
+    if (d->pred.steps[i].comb < 0 || d->pred.steps[i].comb > 2 || d->pred.steps[i].cmp < 0 || d->pred.steps[i].cmp > 4)
+      return fail(c, LJ_EINVAL, "predicate step %d has an unknown operator", i);
+  }
+  if (d->n_rows > 0 && d->pred.n_steps > 0 && !d->attr) return fail(c, LJ_EINVAL, "predicate needs the attr column");
+  if (d->n_rows > 0 && !d->payload && !d->attr) return fail(c, LJ_EINVAL, "PHT_2 needs a payload or attr column");
+  if (d->ht_kind != LJ_HT_PHT2) return fail(c, LJ_EUNSUPPORTED, "ht_kind %d not supported", d->ht_kind);
+  if (d->ht_len < 1 || d->ht_len > (1ll << 30)) return fail(c, LJ_EINVAL, "ht_len %lld out of range [1, 2^30]", (long long)d->ht_len);
+  if (d->bloom_hash == LJ_BLOOM_MOD) {
+    if (d->bloom_k != 1) return fail(c, LJ_EINVAL, "MOD Bloom hash supports k = 1 only");
+    if (d->bloom_bits < 32 || d->bloom_bits > (1ll << 31)) return fail(c, LJ_EINVAL, "bloom_bits %lld out of range", (long long)d->bloom_bits);
+  } else if (d->bloom_hash == LJ_BLOOM_MULT) {
+    if (d->bloom_k < 1 || d->bloom_k > 3) return fail(c, LJ_EINVAL, "bloom_k %d not in 1..3", d->bloom_k);
+    if (d->bloom_bits < 64 || d->bloom_bits > (1ll << 31) || (d->bloom_bits & (d->bloom_bits - 1)))
+      return fail(c, LJ_EINVAL, "MULT Bloom needs a power-of-two bloom_bits in [64, 2^31]");
+    if (d->bloom_layout != LJ_BLOOM_STANDARD && d->bloom_layout != LJ_BLOOM_BLOCKED)
+      return fail(c, LJ_EINVAL, "bloom_layout %d unknown", d->bloom_layout);
+  } else if (d->bloom_hash != LJ_BLOOM_NONE) {
+    return fail(c, LJ_EINVAL, "bloom_hash %d unknown", d->bloom_hash);
+  }
+  return LJ_OK;
+}
+
+static void dim_hashes(lj_dim* D) {
+  const lj_dim_desc& d = D->desc;
+  D->ht_len = make_fastmod((uint32_t)d.ht_len);
+  D->ht_words = 2 * d.ht_len;
+  BloomHash& b = D->bh;
+  b = BloomHash{};
+  b.kind = BLOOM_NONE;
+  b.keys_min = d.keys_min;
+  D->bloom_words = 0;
+  if (d.bloom_hash == LJ_BLOOM_MOD) {
+    D->bloom_words = (d.bloom_bits + 31) / 32;
+    const uint64_t m = (uint64_t)D->bloom_words * 32;
+    b.k = 1;
+    b.mod = make_fastmod((uint32_t)m);
+    b.kind = (m & (m - 1)) == 0 ? BLOOM_MOD_POW2 : BLOOM_MOD;
+  } else if (d.bloom_hash == LJ_BLOOM_MULT) {
+    D->bloom_words = d.bloom_bits / 32;
+    const int lg = ilog2_u64((uint64_t)d.bloom_bits);
+    b.k = d.bloom_k;
+    if (d.bloom_layout == LJ_BLOOM_BLOCKED) {
+      b.kind = BLOOM_MULT_BLOCKED;
+      b.shift = (uint32_t)(32 - (lg - 5));
+    } else {
+      b.kind = BLOOM_MULT_STD;
+      b.shift = (uint32_t)(32 - lg);
+    }
+  }
+}
+
+static void dim_free(lj_ctx* c, lj_dim* D) {
+  if (!D) return;
+  dev_release(c, D->ht);
+  dev_release(c, D->bloom);
+  dev_release(c, D->counters);
+  delete D;
+}
+
+// Allocate the tables of a dimension (no launches).
+static lj_status dim_alloc(lj_ctx* c, const lj_dim_desc* desc, lj_dim** out) {
+  lj_status s = check_dim_desc(c, desc);
+  if (s != LJ_OK) return s;
+  lj_dim* D = new lj_dim();
+  D->desc = *desc;
+  if (!D->desc.payload) D->desc.payload = D->desc.attr;
+  dim_hashes(D);
+  if ((s = dev_alloc_t(c, D->ht_words, &D->ht)) != LJ_OK ||
+      (D->bloom_words && (s = dev_alloc_t(c, D->bloom_words, &D->bloom)) != LJ_OK) ||
+      (s = dev_alloc_t(c, 4, &D->counters)) != LJ_OK) {
+    dim_free(c, D);
+    return s;
+  }
+  *out = D;
+  return LJ_OK;
+}
+
+static void add_fill(FillArgs& f, void* p, int64_t words) {
+  if (!p || words <= 0) return;
+  f.r[f.n].ptr = static_cast<uint32_t*>(p);
+  f.r[f.n].words = words;
+  f.r[f.n].value = 0;
+  f.n++;
+}
+
+// Queue fill + build of up to 4 allocated dimensions.
+static lj_status dims_launch(lj_ctx* c, lj_dim* const* dims, int n, hipEvent_t after_fill) {
+  FillArgs f{};
+  BuildArgs b{};
+  int64_t blocks = 0;
+  const int64_t tile = build_tile_rows();
+  for (int i = 0; i < n; i++) {
+    lj_dim* D = dims[i];
+    add_fill(f, D->ht, D->ht_words);
+    add_fill(f, D->bloom, D->bloom_words);
+    add_fill(f, D->counters, 8);
+    BuildDim& bd = b.dim[b.n_dims];
+    bd.keys = D->desc.keys;
+    bd.attr = D->desc.attr;
+    bd.payload = D->desc.payload;
+    bd.n = D->desc.n_rows;
+    bd.pred.n_steps = D->desc.pred.n_steps;
+    for (int k = 0; k < 4; k++) {
+      bd.pred.comb[k] = D->desc.pred.steps[k].comb;
+      bd.pred.cmp[k] = D->desc.pred.steps[k].cmp;
+      bd.pred.value[k] = D->desc.pred.steps[k].value;
+    }
+    bd.ht = D->ht;
+    bd.ht_len = D->ht_len;
+    bd.keys_min = D->desc.keys_min;
+    bd.bloom = D->bloom;
+    bd.bh = D->bh;
+    bd.counters = D->counters;
+    bd.block_begin = blocks;
+    blocks += (D->desc.n_rows + tile - 1) / tile;
+    b.n_dims++;
+  }
+  HIP_TRY(c, launch_fill(f, c->stream));
+  if (after_fill) HIP_TRY(c, hipEventRecord(after_fill, c->stream));
+  // workgroup w builds the last dimension whose block_begin <= w (an empty
+  // dimension owns an empty range)
+  if (blocks > 0) HIP_TRY(c, launch_build(b, blocks, c->stream));
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_dim_build(lj_ctx* c, const lj_dim_desc* desc, lj_dim** out) {
+  if (!c || !out) return fail(c, LJ_EINVAL, "NULL argument");
+  *out = nullptr;
+  HIP_TRY(c, hipSetDevice(c->device));
+  lj_dim* D = nullptr;
+  lj_status s = dim_alloc(c, desc, &D);
+  if (s != LJ_OK) return s;
+  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  if ((s = dims_launch(c, &D, 1, nullptr)) != LJ_OK) {
+    dim_free(c, D);
+    return s;
+  }
+  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+  unsigned long long h[2] = {0, 0};
+  HIP_TRY(c, hipMemcpyAsync(h, D->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipEventElapsedTime(&D->time_build_ms, c->ev[0], c->ev[1]));
+  D->n_selected = (int64_t)h[0];
+  D->n_oob = (int64_t)h[1];
+  *out = D;
+  if (D->n_oob > 0)
+    return fail(c, LJ_EDOMAIN, "%lld selected keys are below keys_min (tables built without them)", (long long)D->n_oob);
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_dim_get_info(lj_ctx* c, const lj_dim* D, lj_dim_info* info) {
+  if (!c || !D || !info) return fail(c, LJ_EINVAL, "NULL argument");
+  HIP_TRY(c, hipSetDevice(c->device));
+  memset(info, 0, sizeof *info);
+  info->n_selected = D->n_selected;
+  info->n_oob = D->n_oob;
+  info->ht_words = D->ht_words;
+  info->bloom_words = D->bloom_words;
+  info->time_build_ms = D->time_build_ms;
+  if (D->bloom_words) {
+    unsigned long long* cnt = D->counters + 2;
+    HIP_TRY(c, hipMemsetAsync(cnt, 0, 8, c->stream));
+    HIP_TRY(c, launch_popcount(D->bloom, D->bloom_words, cnt, c->stream));
+    unsigned long long h = 0;
+    HIP_TRY(c, hipMemcpyAsync(&h, cnt, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    info->bloom_bits_set = (int64_t)h;
+  }
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_dim_export(lj_ctx* c, const lj_dim* D, int32_t* ht_host, int64_t ht_words,
+                                   uint32_t* bloom_host, int64_t bloom_words) {
+  if (!c || !D) return fail(c, LJ_EINVAL, "NULL argument");
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (ht_host) {
+    if (ht_words != D->ht_words) return fail(c, LJ_EINVAL, "ht_words %lld != %lld", (long long)ht_words, (long long)D->ht_words);
+    HIP_TRY(c, hipMemcpyAsync(ht_host, D->ht, (size_t)ht_words * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  if (bloom_host) {
+    if (bloom_words != D->bloom_words)
+      return fail(c, LJ_EINVAL, "bloom_words %lld != %lld", (long long)bloom_words, (long long)D->bloom_words);
+    HIP_TRY(c, hipMemcpyAsync(bloom_host, D->bloom, (size_t)bloom_words * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return LJ_OK;
+}
+
+extern "C" void lj_dim_release(lj_ctx* c, lj_dim* D) {
+  if (!c || !D) return;
+  (void)hipStreamSynchronize(c->stream);
+  dim_free(c, D);
+}
+
+// ------------------------------------------------------------------ queries --
+static lj_status ensure_groups(lj_ctx* c, int64_t n_groups) {
+  if (n_groups <= c->g_cap) return LJ_OK;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  dev_release(c, c->gsum);
+  dev_release(c, c->gcount);
+  dev_release(c, c->gprod);
+  dev_release(c, c->gpay);
+  dev_release(c, c->touched);
+  dev_release(c, c->out);
+  c->g_cap = 0;
+  const int64_t cap = n_groups;
+  const uint32_t tcap = (uint32_t)std::min<int64_t>(n_groups, 1 << 20);
+  lj_status s;
+  if ((s = dev_alloc_t(c, cap, &c->gsum)) != LJ_OK || (s = dev_alloc_t(c, cap, &c->gcount)) != LJ_OK ||
+      (s = dev_alloc_t(c, cap, &c->gprod)) != LJ_OK || (s = dev_alloc_t(c, cap * kMaxDims, &c->gpay)) != LJ_OK ||
+      (s = dev_alloc_t(c, (int64_t)tcap, &c->touched)) != LJ_OK || (s = dev_alloc_t(c, (int64_t)tcap, &c->out)) != LJ_OK)
+    return s;
+  if (!c->n_touched) {
+    if ((s = dev_alloc_t(c, 64, &c->n_touched)) != LJ_OK) return s;
+    if ((s = dev_alloc_t(c, 1 + kStats, &c->out_hdr)) != LJ_OK) return s;
+  }
+  FillArgs f{};
+  add_fill(f, c->gsum, cap * 2);
+  add_fill(f, c->gcount, cap * 2);
+  add_fill(f, c->gprod, cap * 2);
+  add_fill(f, c->n_touched, 64);
+  HIP_TRY(c, launch_fill(f, c->stream));
+  c->g_cap = cap;
+  c->touched_cap = tcap;
+  return LJ_OK;
+}
+
+static lj_status ensure_stats(lj_ctx* c, int blocks) {
+  if (blocks <= c->stats_cap) return LJ_OK;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  dev_release(c, c->stats);
+  c->stats = nullptr;
+  c->stats_cap = 0;
+  lj_status s = dev_alloc_t(c, (int64_t)blocks * kStats, &c->stats);
+  if (s != LJ_OK) return s;
+  c->stats_cap = blocks;
+  return LJ_OK;
+}
+
+static bool is_perm(const int32_t* o, int n) {
+  int seen = 0;
+  for (int i = 0; i < n; i++) {
+    if (o[i] < 0 || o[i] >= n || (seen >> o[i]) & 1) return false;
+    seen |= 1 << o[i];
+  }
+  return true;
+}
+
+struct QueryOut {
+  std::vector<GroupOut> groups;
+  long long stats[kStats];
+  float t_fill = 0, t_build = 0, t_probe = 0, t_total = 0;
+  int64_t dim_selected[kMaxDims];
+  int64_t dim_oob[kMaxDims];
+};
+
+// The common path of lj_star_join and lj_hash_join.
+static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, QueryOut& qo) {
+  if (!q) return fail(c, LJ_EINVAL, "query descriptor is NULL");
+  const int nd = q->n_dims;
+  if (nd < 1 || nd > LJ_MAX_DIMS) return fail(c, LJ_EINVAL, "n_dims %d not in 1..%d", nd, LJ_MAX_DIMS);
+  if (q->n_fact < 0 || q->n_fact >= (1ll << 31) - 4096) return fail(c, LJ_EINVAL, "n_fact %lld out of range", (long long)q->n_fact);
+  if (q->method < LJ_METHOD_NAIVE || q->method > LJ_METHOD_LIP_ADAPTIVE) return fail(c, LJ_EINVAL, "unknown method %d", q->method);
+  if (!is_perm(q->order, nd)) return fail(c, LJ_EINVAL, "order is not a permutation of 0..%d", nd - 1);
+  int32_t bloom_order[LJ_MAX_DIMS];
+  bool bo_default = true;
+  for (int i = 0; i < nd; i++) bo_default = bo_default && q->bloom_order[i] < 0;
+  for (int i = 0; i < nd; i++) bloom_order[i] = bo_default ? q->order[i] : q->bloom_order[i];
+  if (!is_perm(bloom_order, nd)) return fail(c, LJ_EINVAL, "bloom_order is not a permutation");
+  if (q->n_groups < 1 || q->n_groups > (1ll << 26)) return fail(c, LJ_EINVAL, "n_groups %lld not in [1, 2^26]", (long long)q->n_groups);
+  if (q->n_fact > 0) {
+    if (!q->fact_measure || ((uintptr_t)q->fact_measure & 15)) return fail(c, LJ_EINVAL, "fact_measure must be a 16-byte aligned device pointer");
+    for (int d = 0; d < nd; d++)
+      if (!q->fact_fk[d] || ((uintptr_t)q->fact_fk[d] & 15))
+        return fail(c, LJ_EINVAL, "fact_fk[%d] must be a 16-byte aligned device pointer", d);
+  }
+  HIP_TRY(c, hipSetDevice(c->device));
+
+  // dimensions: resident, or built inside the call
+  lj_dim* dims[LJ_MAX_DIMS] = {nullptr, nullptr, nullptr, nullptr};
+  lj_dim* owned[LJ_MAX_DIMS];
+  int n_owned = 0;
+  lj_status s = LJ_OK;
+  for (int d = 0; d < nd && s == LJ_OK; d++) {
+    if (q->resident[d]) {
+      dims[d] = const_cast<lj_dim*>(q->resident[d]);
+    } else {
+      s = dim_alloc(c, &q->dims[d], &dims[d]);
+      if (s == LJ_OK) owned[n_owned++] = dims[d];
+    }
+  }
+  auto cleanup = [&]() {
+    for (int i = 0; i < n_owned; i++) dim_free(c, owned[i]);
+  };
+  if (s != LJ_OK) {
+    cleanup();
+    return s;
+  }
+  if ((s = ensure_groups(c, q->n_groups)) != LJ_OK) {
+    cleanup();
+    return s;
+  }
+
+  // ---- probe plan: workgroup shape, LDS-resident filters, grid
+  const bool lip = q->method != LJ_METHOD_NAIVE;
+  const bool adaptive = q->method == LJ_METHOD_LIP_ADAPTIVE;
+  ProbeConfig cfg;
+  cfg.wg = c->probe_wg ? c->probe_wg : 1024;
+  cfg.vec = c->probe_vec ? c->probe_vec : 2;
+  const int lds_policy = c->lds_policy >= 0 ? c->lds_policy : 0;
+  ProbeArgs a{};
+  int lds_words = 0;
+  const int64_t lds_budget_words = ((int64_t)c->max_lds - (int64_t)probe_lds_bytes(cfg, 0) - 16) / 4;
+  for (int d = 0; d < nd; d++) {
+    ProbeDim& P = a.dim[d];
+    lj_dim* D = dims[d];
+    P.fk = q->fact_fk[d];
+    P.bloom = D->bloom;
+    P.ht = D->ht;
+    P.bh = lip ? D->bh : BloomHash{};
+    if (!lip) P.bh.kind = BLOOM_NONE;
+    P.ht_len = D->ht_len;
+    P.keys_min = D->desc.keys_min;
+    P.bloom_lds = -1;
+    P.bloom_words = (int32_t)D->bloom_words;
+    P.gmult = q->group_mult[d];
+  }
+  if (lip) {
+    // place filters in LDS in load order; policy 1 keeps only the first one
+    for (int st = 0; st < nd; st++) {
+      const int d = adaptive ? st : bloom_order[st];
+      ProbeDim& P = a.dim[d];
+      if (P.bh.kind == BLOOM_NONE || !P.bloom) continue;
+      const int64_t w4 = (P.bloom_words + 3) & ~3ll;
+      if (lds_words + w4 > lds_budget_words) continue;
+      if (lds_policy == 1 && !adaptive && st > 0) continue;
+      P.bloom_lds = lds_words;
+      lds_words += (int)w4;
+    }
+  }
+  const size_t lds_bytes = probe_lds_bytes(cfg, lds_words);
+  int per_cu = 0;
+  HIP_TRY(c, probe_occupancy(cfg, lds_bytes, &per_cu));
+  if (per_cu < 1) {
+    cleanup();
+    return fail(c, LJ_EUNSUPPORTED, "probe configuration wg=%d vec=%d with %zu B LDS does not fit", cfg.wg, cfg.vec, lds_bytes);
+  }
+  const int64_t tile_rows = (int64_t)cfg.vec * kSegRows;
+  const int nw = cfg.wg / 64;
+  int64_t grid = (int64_t)c->n_cu * per_cu;
+  const int64_t need = (q->n_fact + tile_rows * nw - 1) / (tile_rows * nw);
+  if (grid > need) grid = need;
+  if (grid < 1) grid = 1;
+  if ((s = ensure_stats(c, (int)grid)) != LJ_OK) {
+    cleanup();
+    return s;
+  }
+
+  a.measure = q->fact_measure;
+  a.n = q->n_fact;
+  a.n_dims = nd;
+  a.method = lip ? 1 : 0;
+  a.adaptive = adaptive ? 1 : 0;
+  a.want_product = want_product;
+  for (int i = 0; i < LJ_MAX_DIMS; i++) {
+    a.order[i] = i < nd ? q->order[i] : i;
+    a.load_order[i] = i < nd ? (lip ? bloom_order[i] : q->order[i]) : i;
+  }
+  a.lds_filter_words = lds_words;
+  if (adaptive) {
+    const int64_t total_waves = grid * nw;
+    const int64_t nr = (q->n_fact + kSegRows - 1) / kSegRows * kSegRows;
+    a.sample_rows = std::min<int64_t>(total_waves * kSegRows, nr);
+  }
+  a.goffset = q->group_offset;
+  a.n_groups = q->n_groups;
+  a.gsum = c->gsum;
+  a.gcount = c->gcount;
+  a.gprod = c->gprod;
+  a.gpay = c->gpay;
+  a.touched = c->touched;
+  a.n_touched = c->n_touched;
+  a.touched_cap = c->touched_cap;
+  a.stats = c->stats;
+
+  // ---- queue: fill + build, probe, finalize
+  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  if (n_owned > 0) {
+    if ((s = dims_launch(c, owned, n_owned, c->ev[1])) != LJ_OK) {
+      cleanup();
+      return s;
+    }
+  } else {
+    HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+  }
+  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
+  if (q->n_fact > 0) HIP_TRY(c, launch_probe(cfg, a, (int)grid, lds_bytes, c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  FinalizeArgs fa{};
+  fa.gsum = c->gsum;
+  fa.gcount = c->gcount;
+  fa.gprod = c->gprod;
+  fa.gpay = c->gpay;
+  fa.touched = c->touched;
+  fa.n_touched = c->n_touched;
+  fa.touched_cap = c->touched_cap;
+  fa.n_blocks = q->n_fact > 0 ? (int)grid : 0;
+  fa.stats = c->stats;
+  fa.out_hdr = c->out_hdr;
+  fa.out = c->out;
+  fa.out_cap = c->touched_cap;
+  HIP_TRY(c, launch_finalize(fa, c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev[4], c->stream));
+
+  // ---- results
+  long long* hdr = static_cast<long long*>(c->host_buf);
+  GroupOut* hg = reinterpret_cast<GroupOut*>(hdr + 1 + kStats);
+  HIP_TRY(c, hipMemcpyAsync(hdr, c->out_hdr, sizeof(long long) * (1 + kStats), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(hg, c->out, sizeof(GroupOut) * std::min<size_t>(kHostGroups, c->touched_cap),
+                            hipMemcpyDeviceToHost, c->stream));
+  unsigned long long cnt_h[kMaxDims][2];
+  for (int d = 0; d < nd; d++) {
+    if (!q->resident[d])
+      HIP_TRY(c, hipMemcpyAsync(cnt_h[d], dims[d]->counters, sizeof cnt_h[d], hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const int64_t n_touched = hdr[0];
+  memcpy(qo.stats, hdr + 1, sizeof qo.stats);
+  if (n_touched > (int64_t)c->touched_cap) {
+    cleanup();
+    return fail(c, LJ_EUNSUPPORTED, "%lld groups were hit; at most %u can be listed", (long long)n_touched, c->touched_cap);
+  }
+  qo.groups.assign(hg, hg + std::min<int64_t>(n_touched, kHostGroups));
+  if (n_touched > kHostGroups) {
+    qo.groups.resize((size_t)n_touched);
+    HIP_TRY(c, hipMemcpy(qo.groups.data() + kHostGroups, c->out + kHostGroups,
+                         sizeof(GroupOut) * (size_t)(n_touched - kHostGroups), hipMemcpyDeviceToHost));
+  }
+  std::sort(qo.groups.begin(), qo.groups.end(), [](const GroupOut& x, const GroupOut& y) { return x.g < y.g; });
+  HIP_TRY(c, hipEventElapsedTime(&qo.t_fill, c->ev[0], c->ev[1]));
+  HIP_TRY(c, hipEventElapsedTime(&qo.t_build, c->ev[1], c->ev[2]));
+  HIP_TRY(c, hipEventElapsedTime(&qo.t_probe, c->ev[2], c->ev[3]));  // the fused probe kernel alone
+  HIP_TRY(c, hipEventElapsedTime(&qo.t_total, c->ev[0], c->ev[4]));
+  int64_t build_oob = 0;
+  for (int d = 0; d < nd; d++) {
+    if (q->resident[d]) {
+      qo.dim_selected[d] = dims[d]->n_selected;
+      qo.dim_oob[d] = dims[d]->n_oob;
+    } else {
+      qo.dim_selected[d] = (int64_t)cnt_h[d][0];
+      qo.dim_oob[d] = (int64_t)cnt_h[d][1];
+    }
+    build_oob += qo.dim_oob[d];
+  }
+  cleanup();
+  if (build_oob > 0 || qo.stats[ST_OOB] > 0)
+    return fail(c, LJ_EDOMAIN, "%lld dimension keys and %lld fact keys/groups fell outside their hash domains",
+                (long long)build_oob, (long long)qo.stats[ST_OOB]);
+  return LJ_OK;
+}
+
+static void fill_stats(const QueryOut& qo, int nd, lj_probe_stats* ps) {
+  memset(ps, 0, sizeof *ps);
+  for (int s = 0; s <= LJ_MAX_DIMS; s++) {
+    ps->rows_in[s] = qo.stats[ST_ROWS_IN + s];
+    ps->lines_in[s] = qo.stats[ST_LINES_IN + s];
+  }
+  ps->rows_bloom_pass = qo.stats[ST_BLOOM_PASS];
+  ps->sample_rows = qo.stats[ST_SAMPLE_ROWS];
+  ps->sample_lines = qo.stats[ST_SAMPLE_LINES];
+  ps->n_oob = qo.stats[ST_OOB];
+  for (int s = 0; s < LJ_MAX_DIMS; s++) ps->order_used[s] = s < nd ? (int32_t)((qo.stats[ST_ORDER] >> (8 * s)) & 0xff) : -1;
+}
+
+extern "C" lj_status lj_star_join(lj_ctx* c, const lj_star_desc* q, lj_star_result* out) {
+  if (!c || !q || !out) return fail(c, LJ_EINVAL, "NULL argument");
+  QueryOut qo;
+  lj_status s = run_query(c, q, 0, qo);
+  if (s != LJ_OK && s != LJ_EDOMAIN) return s;
+  out->n_groups = (int64_t)qo.groups.size();
+  out->n_rows = 0;
+  out->sum_total = 0;
+  for (size_t i = 0; i < qo.groups.size(); i++) {
+    const GroupOut& g = qo.groups[i];
+    out->n_rows += (int64_t)g.count;
+    out->sum_total += g.sum;
+    if ((int64_t)i < out->groups_cap && out->groups) {
+      lj_group& o = out->groups[i];
+      o.g = g.g;
+      for (int d = 0; d < LJ_MAX_DIMS; d++) o.payload[d] = d < q->n_dims ? g.pay[d] : 0;
+      o.sum = g.sum;
+      o.count = (int64_t)g.count;
+      o.sum_i32 = (int32_t)(uint32_t)(uint64_t)g.sum;
+      o.reserved0 = 0;
+    }
+  }
+  out->time_query_ms = qo.t_total;
+  out->time_build_ms = qo.t_fill + qo.t_build;
+  out->time_probe_ms = qo.t_probe;
+  out->reserved0 = 0;
+  fill_stats(qo, q->n_dims, &out->stats);
+  return s;
+}
+
+extern "C" lj_status lj_hash_join(lj_ctx* c, const lj_join_desc* j, lj_join_result* out) {
+  if (!c || !j || !out) return fail(c, LJ_EINVAL, "NULL argument");
+  if (j->method != LJ_METHOD_NAIVE && j->method != LJ_METHOD_LIP && j->method != LJ_METHOD_LIP_ADAPTIVE)
+    return fail(c, LJ_EINVAL, "unknown method %d", j->method);
+  lj_star_desc q;
+  memset(&q, 0, sizeof q);
+  q.fact_fk[0] = j->fact_fk;
+  q.fact_measure = j->fact_measure;
+  q.n_fact = j->n_fact;
+  q.n_dims = 1;
+  q.method = j->method;
+  q.order[0] = 0;
+  for (int i = 0; i < LJ_MAX_DIMS; i++) q.bloom_order[i] = -1;
+  q.dims[0] = j->dim;
+  q.n_groups = 1;
+  QueryOut qo;
+  lj_status s = run_query(c, &q, 1, qo);
+  if (s != LJ_OK && s != LJ_EDOMAIN) return s;
+  memset(out, 0, sizeof *out);
+  out->n_selected = qo.dim_selected[0];
+  if (!qo.groups.empty()) {
+    out->n_match = (int64_t)qo.groups[0].count;
+    out->sum_measure = qo.groups[0].sum;
+    out->sum_product = qo.groups[0].prod;
+  }
+  out->n_bloom_pass = j->method == LJ_METHOD_NAIVE ? j->n_fact : qo.stats[ST_BLOOM_PASS];
+  out->time_build = qo.t_build;
+  out->time_probe = qo.t_probe;
+  out->time_extra = qo.t_fill;
+  out->time_total = qo.t_total;
+  fill_stats(qo, 1, &out->stats);
+  return s;
+}
+
+// ---------------------------------------------------------------- generators --
+extern "C" lj_status lj_gen_host(int32_t kind, uint64_t seed, uint64_t stream, int64_t lo, int64_t hi, int64_t first,
+                                 int64_t n, int32_t* out) {
+  if (n < 0 || (n > 0 && !out) || kind < 0 || kind > 4) return LJ_EINVAL;
+  if (kind == GEN_UNIFORM && (hi < lo || hi - lo + 1 > (1ll << 32))) return LJ_EINVAL;
+  if (kind == GEN_DATEKEY && (hi < 1 || hi > (1ll << 32))) return LJ_EINVAL;
+  const uint64_t key = stream_key(seed, stream);
+  unsigned nt = std::thread::hardware_concurrency();
+  if (nt == 0) nt = 1;
+  if (nt > 32) nt = 32;
+  if (n < (1 << 16)) nt = 1;
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; t++) {
+    const int64_t b = n * t / nt, e = n * (t + 1) / nt;
+    auto work = [=]() {
+      for (int64_t i = b; i < e; i++) out[i] = gen_value(kind, key, lo, hi, first + i);
+    };
+    if (nt == 1) work();
+    else th.emplace_back(work);
+  }
+  for (auto& x : th) x.join();
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_gen_device(lj_ctx* c, int32_t kind, uint64_t seed, uint64_t stream, int64_t lo, int64_t hi,
+                                   int64_t first, int64_t n, int32_t* out_dev) {
+  if (!c) return LJ_EINVAL;
+  if (n < 0 || (n > 0 && !out_dev) || kind < 0 || kind > 4) return fail(c, LJ_EINVAL, "bad generator arguments");
+  if (kind == GEN_UNIFORM && (hi < lo || hi - lo + 1 > (1ll << 32))) return fail(c, LJ_EINVAL, "bad uniform range");
+  if (kind == GEN_DATEKEY && (hi < 1 || hi > (1ll << 32))) return fail(c, LJ_EINVAL, "bad day range");
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, launch_gen(kind, stream_key(seed, stream), lo, hi, first, n, out_dev, c->stream));
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_gen_shuffle_host(uint64_t seed, uint64_t stream, int64_t n, int32_t* out) {
+  if (n < 0 || (n > 0 && !out)) return LJ_EINVAL;
+  const uint64_t key = stream_key(seed, stream);
+  // Durstenfeld's Fisher-Yates: swap out[i] with out[j], j uniform in [0, i]
+  for (int64_t i = n - 1; i > 0; i--) {
+    const int64_t j = (int64_t)bounded_at(key, (uint64_t)i, (uint64_t)(i + 1));
+    const int32_t t = out[i];
+    out[i] = out[j];
+    out[j] = t;
+  }
+  return LJ_OK;
+}

@@ -1,0 +1,146 @@
+// lj_internal.hpp -- kernel argument blocks shared by the kernels and the host
+// code of liblipjoin.so (not part of the public ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lipjoin/block/hash.hpp"
+#include "lipjoin/block/pred.hpp"
+
+namespace lj {
+
+constexpr int kMaxDims = 4;
+constexpr int kGroupSlots = 64;      // LDS-privatised groups per workgroup
+constexpr int kQueue = 64;           // survivor queue entries per wave
+constexpr int kStats = 16;           // per-workgroup statistics words
+constexpr int kSegRows = 256;        // rows of one wave segment (64 lanes x 4)
+
+// statistics word indices
+enum StatIdx : int {
+  ST_ROWS_IN = 0,        // [0..4]
+  ST_LINES_IN = 5,       // [5..9]
+  ST_BLOOM_PASS = 10,
+  ST_SAMPLE_ROWS = 11,
+  ST_SAMPLE_LINES = 12,
+  ST_OOB = 13,
+  ST_ORDER = 14          // order of workgroup 0, 8 bits per stage
+};
+
+struct ProbeDim {
+  const int32_t* fk;      // fact FK column of this dimension
+  const uint32_t* bloom;  // global Bloom words (nullptr: no filter)
+  const int32_t* ht;      // PHT_2 pairs
+  BloomHash bh;
+  FastMod ht_len;
+  int32_t keys_min;       // PHT's HASH Z argument
+  int32_t bloom_lds;      // word offset of the staged copy in LDS, -1 = global
+  int32_t bloom_words;
+  int32_t pad0;
+  int64_t gmult;
+};
+
+struct ProbeArgs {
+  ProbeDim dim[kMaxDims];
+  const int32_t* measure;
+  int64_t n;               // fact rows (< 2^31)
+  int32_t n_dims;
+  int32_t method;          // 0 NAIVE, 1 LIP
+  int32_t adaptive;        // LIP: order chosen from the sampling pass
+  int32_t want_product;    // also sum (int32)(measure * payload of dim 0)
+  int32_t order[kMaxDims];        // hash-probe (join) order
+  int32_t load_order[kMaxDims];   // column load order (NAIVE: order, LIP: Bloom order)
+  int32_t lds_filter_words;
+  int32_t pad0;
+  int64_t sample_rows;     // rows [0, sample_rows) are handled by the sampling pass
+  int64_t goffset;
+  int64_t n_groups;
+  unsigned long long* gsum;
+  unsigned long long* gcount;
+  unsigned long long* gprod;
+  int32_t* gpay;           // [n_groups * 4]
+  uint32_t* touched;       // first-touched groups, in touch order
+  uint32_t* n_touched;
+  uint32_t touched_cap;
+  int32_t pad1;
+  long long* stats;        // [gridDim.x * kStats]
+};
+
+struct BuildDim {
+  const int32_t* keys;
+  const int32_t* attr;
+  const int32_t* payload;
+  int64_t n;
+  PredProgram pred;
+  int32_t* ht;
+  FastMod ht_len;
+  int32_t keys_min;
+  int32_t pad0;
+  uint32_t* bloom;         // nullptr: no filter
+  BloomHash bh;
+  unsigned long long* counters;  // [0] selected rows, [1] keys outside the domain
+  int64_t block_begin;
+};
+
+struct BuildArgs {
+  BuildDim dim[kMaxDims];
+  int32_t n_dims;
+  int32_t pad0;
+};
+
+struct FillRegion {
+  uint32_t* ptr;
+  int64_t words;
+  uint32_t value;
+  int32_t pad0;
+};
+
+constexpr int kMaxFill = 24;
+struct FillArgs {
+  FillRegion r[kMaxFill];
+  int32_t n;
+  int32_t pad0;
+};
+
+// One group of the finalized result (device layout).
+struct GroupOut {
+  long long g;
+  long long sum;
+  unsigned long long count;
+  unsigned long long prod;
+  int32_t pay[kMaxDims];
+};
+
+struct FinalizeArgs {
+  unsigned long long* gsum;
+  unsigned long long* gcount;
+  unsigned long long* gprod;
+  const int32_t* gpay;
+  const uint32_t* touched;
+  uint32_t* n_touched;
+  uint32_t touched_cap;
+  int32_t n_blocks;        // probe workgroups whose stats to reduce
+  const long long* stats;
+  long long* out_hdr;      // [0] n_touched, [1..kStats] reduced stats
+  GroupOut* out;
+  int64_t out_cap;
+};
+
+// Host launchers (lj_kernels_*.hip).
+hipError_t launch_fill(const FillArgs& a, hipStream_t s);
+hipError_t launch_build(const BuildArgs& a, int64_t n_blocks, hipStream_t s);
+hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s);
+hipError_t launch_gen(int32_t kind, uint64_t key, int64_t lo, int64_t hi, int64_t first, int64_t n,
+                      int32_t* out, hipStream_t s);
+hipError_t launch_popcount(const uint32_t* words, int64_t n, unsigned long long* out, hipStream_t s);
+
+// Probe kernel configurations.
+struct ProbeConfig {
+  int wg;      // threads per workgroup (256, 512, 1024)
+  int vec;     // 256-row segments per wave tile (1, 2, 4)
+};
+size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words);
+hipError_t probe_occupancy(const ProbeConfig& c, size_t lds_bytes, int* blocks_per_cu);
+hipError_t launch_probe(const ProbeConfig& c, const ProbeArgs& a, int grid, size_t lds_bytes, hipStream_t s);
+
+}  // namespace lj

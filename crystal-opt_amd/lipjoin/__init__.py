@@ -1,0 +1,250 @@
+"""lipjoin -- host API of the MI355X star-join engine (ctypes over liblipjoin.so).
+
+Mirrors the reference's host drivers for this path:
+  * Context.hash_join  <- TimeKeeper hashJoin(...)  src/ops/join.cu:117-148,
+                                                      src/ops/bloom_join.cu:121-156
+  * Context.star_join  <- float runQuery(...)       src/ssb-jo/q33/0-lip.cu:205-302
+  * Context.build_dim  <- build_kernel / build_hashtable_{s,c,d}
+Device columns are passed as torch tensors (int32, on the context's device) or
+as raw device addresses.  There is no CPU fallback: every call runs the HIP
+kernels of liblipjoin.so, and a missing library raises LibraryMissing.
+"""
+import ctypes as C
+
+from . import _abi
+from ._abi import (  # noqa: F401  (re-exported constants)
+    BLOOM_BLOCKED, BLOOM_MOD, BLOOM_MULT, BLOOM_NONE, BLOOM_STANDARD, CMP_EQ, CMP_GT, CMP_GTE,
+    CMP_LT, CMP_LTE, COMB_AND, COMB_OR, COMB_SET, GEN_DATEKEY, GEN_DATEKEY_SEQ, GEN_SEQUENCE,
+    GEN_UNIFORM, GEN_YEAR_OF_DAY, METHOD_LIP, METHOD_LIP_ADAPTIVE, METHOD_NAIVE, MAX_DIMS,
+    LibraryMissing,
+)
+
+METHODS = {"naive": METHOD_NAIVE, "lip": METHOD_LIP, "lip-adaptive": METHOD_LIP_ADAPTIVE}
+
+
+class LJError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_abi.STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def lib():
+    return _abi.load()
+
+
+def _ptr(x):
+    """Device address of a torch tensor, or an int address, or None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def round_up_pow2(n):
+    """roundUpToPowerOfTwo of src/ssb/ssb_utils.h:85-96 (0 -> 1)."""
+    n = int(n)
+    if n == 0:
+        return 1
+    n -= 1
+    for s in (1, 2, 4, 8, 16):
+        n |= n >> s
+    return (n + 1) & 0xFFFFFFFF
+
+
+def pred(*steps):
+    """Predicate program: steps of (comb, cmp, value); () selects every row."""
+    p = _abi.Pred()
+    if len(steps) > _abi.MAX_PRED_STEPS:
+        raise ValueError("at most 4 predicate steps")
+    p.n_steps = len(steps)
+    for i, (comb, cmp, val) in enumerate(steps):
+        p.steps[i].comb, p.steps[i].cmp, p.steps[i].value = comb, cmp, val
+    return p
+
+
+def dim_desc(keys, n_rows, ht_len, attr=None, payload=None, predicate=(), keys_min=0,
+             bloom=None):
+    """Dimension descriptor.  bloom: None, ("mod", bits) or ("mult", bits, k, layout)."""
+    d = _abi.DimDesc()
+    d.keys, d.attr, d.payload = _ptr(keys), _ptr(attr), _ptr(payload)
+    d.n_rows = int(n_rows)
+    d.pred = predicate if isinstance(predicate, _abi.Pred) else pred(*predicate)
+    d.ht_kind = _abi.HT_PHT2
+    d.keys_min = int(keys_min)
+    d.ht_len = int(ht_len)
+    d.bloom_hash, d.bloom_layout, d.bloom_k, d.bloom_bits = BLOOM_NONE, 0, 0, 0
+    if bloom is not None:
+        kind = bloom[0]
+        if kind == "mod":
+            d.bloom_hash, d.bloom_k, d.bloom_bits = BLOOM_MOD, 1, int(bloom[1])
+        elif kind == "mult":
+            d.bloom_hash, d.bloom_bits, d.bloom_k = BLOOM_MULT, int(bloom[1]), int(bloom[2])
+            d.bloom_layout = bloom[3] if len(bloom) > 3 else BLOOM_STANDARD
+        else:
+            raise ValueError(f"unknown Bloom kind {kind!r}")
+    return d
+
+
+def stats_dict(s):
+    return {
+        "rows_in": list(s.rows_in), "lines_in": list(s.lines_in),
+        "rows_bloom_pass": s.rows_bloom_pass, "sample_rows": s.sample_rows,
+        "sample_lines": s.sample_lines, "order_used": list(s.order_used), "n_oob": s.n_oob,
+    }
+
+
+class Dim:
+    """A device-resident dimension (PHT_2 + optional Bloom filter)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.handle = ctx, handle
+
+    def info(self):
+        i = _abi.DimInfo()
+        self.ctx._check(lib().lj_dim_get_info(self.ctx.h, self.handle, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in _abi.DimInfo._fields_}
+
+    def export(self):
+        """(PHT words int32[2*ht_len], Bloom words uint32[...]) as numpy arrays."""
+        import numpy as np
+        info = self.info()
+        ht = np.zeros(info["ht_words"], dtype=np.int32)
+        bf = np.zeros(max(info["bloom_words"], 0), dtype=np.uint32)
+        self.ctx._check(lib().lj_dim_export(self.ctx.h, self.handle, ht.ctypes.data, ht.size,
+                                            bf.ctypes.data if bf.size else None, bf.size))
+        return ht, bf
+
+    def release(self):
+        if self.handle:
+            lib().lj_dim_release(self.ctx.h, self.handle)
+            self.handle = None
+
+
+class Context:
+    """One lj_ctx: a device, a stream, a caching allocator and an HBM budget."""
+
+    def __init__(self, device=0, hbm_budget=0, stream=None):
+        L = lib()
+        h = C.c_void_p()
+        st = _abi.OK
+        if stream is None:
+            try:
+                import torch
+                if torch.cuda.is_available():
+                    stream = torch.cuda.current_stream(device).cuda_stream
+            except ImportError:
+                stream = None
+        st = L.lj_create(int(device), int(hbm_budget), stream, C.byref(h))
+        if st != _abi.OK:
+            raise LJError(st, L.lj_last_error(None).decode())
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib().lj_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, allow=()):
+        if st != _abi.OK and st not in allow:
+            raise LJError(st, lib().lj_last_error(self.h).decode())
+        return st
+
+    def sync(self):
+        self._check(lib().lj_sync(self.h))
+
+    def bytes_in_use(self):
+        return lib().lj_bytes_in_use(self.h)
+
+    # -- dimensions -------------------------------------------------------------
+    def build_dim(self, desc):
+        h = C.c_void_p()
+        self._check(lib().lj_dim_build(self.h, C.byref(desc), C.byref(h)))
+        return Dim(self, h)
+
+    # -- single join (hashJoin) -------------------------------------------------------
+    def hash_join(self, fact_fk, fact_measure, n_fact, dim, method="lip", allow_domain=False):
+        j = _abi.JoinDesc()
+        j.fact_fk, j.fact_measure, j.n_fact = _ptr(fact_fk), _ptr(fact_measure), int(n_fact)
+        j.dim = dim
+        j.method = METHODS[method] if isinstance(method, str) else method
+        r = _abi.JoinResult()
+        st = self._check(lib().lj_hash_join(self.h, C.byref(j), C.byref(r)),
+                         allow=(_abi.EDOMAIN,) if allow_domain else ())
+        out = {k: getattr(r, k) for k, _ in _abi.JoinResult._fields_ if k != "stats"}
+        out["stats"] = stats_dict(r.stats)
+        out["status"] = st
+        return out
+
+    # -- star join (runQuery) ---------------------------------------------------------
+    def star_join(self, desc, groups_cap=1 << 16, allow_domain=False):
+        arr = (_abi.Group * groups_cap)()
+        r = _abi.StarResult()
+        r.groups, r.groups_cap = C.cast(arr, C.POINTER(_abi.Group)), groups_cap
+        st = self._check(lib().lj_star_join(self.h, C.byref(desc), C.byref(r)),
+                         allow=(_abi.EDOMAIN,) if allow_domain else ())
+        n = min(r.n_groups, groups_cap)
+        groups = [(arr[i].g, tuple(arr[i].payload[: desc.n_dims]), arr[i].sum, arr[i].count,
+                   arr[i].sum_i32) for i in range(n)]
+        return {
+            "groups": groups, "n_groups": r.n_groups, "n_rows": r.n_rows, "sum_total": r.sum_total,
+            "time_query_ms": r.time_query_ms, "time_build_ms": r.time_build_ms,
+            "time_probe_ms": r.time_probe_ms, "stats": stats_dict(r.stats), "status": st,
+        }
+
+    # -- synthetic columns -------------------------------------------------------------
+    def gen_device(self, out, kind, seed, stream, lo, hi, first=0, n=None):
+        n = out.numel() if n is None else n
+        self._check(lib().lj_gen_device(self.h, kind, seed, stream, lo, hi, first, n, _ptr(out)))
+
+
+def gen_host(kind, seed, stream, lo, hi, n, first=0):
+    """numpy int32 column from the counter-based generator (same bits as gen_device)."""
+    import numpy as np
+    out = np.empty(int(n), dtype=np.int32)
+    st = lib().lj_gen_host(kind, seed, stream, lo, hi, first, int(n), out.ctypes.data if n else None)
+    if st != _abi.OK:
+        raise LJError(st, "lj_gen_host")
+    return out
+
+
+def shuffle_host(arr, seed, stream):
+    """In-place Fisher-Yates shuffle of an int32 numpy array."""
+    st = lib().lj_gen_shuffle_host(seed, stream, arr.size, arr.ctypes.data if arr.size else None)
+    if st != _abi.OK:
+        raise LJError(st, "lj_gen_shuffle_host")
+    return arr
+
+
+def star_desc(fact_fk, fact_measure, n_fact, dims, order, method="lip", bloom_order=None,
+              group_mult=(0, 0, 0, 0), group_offset=0, n_groups=1, resident=None):
+    q = _abi.StarDesc()
+    nd = len(dims) if dims else len(resident)
+    q.n_dims = nd
+    for d in range(nd):
+        q.fact_fk[d] = _ptr(fact_fk[d])
+        if dims and dims[d] is not None:
+            q.dims[d] = dims[d]
+        if resident and resident[d] is not None:
+            q.resident[d] = resident[d].handle
+        q.group_mult[d] = int(group_mult[d])
+    q.fact_measure, q.n_fact = _ptr(fact_measure), int(n_fact)
+    q.method = METHODS[method] if isinstance(method, str) else method
+    for i in range(MAX_DIMS):
+        q.order[i] = order[i] if i < nd else i
+        q.bloom_order[i] = (bloom_order[i] if bloom_order is not None and i < nd else -1)
+    q.group_offset, q.n_groups = int(group_offset), int(n_groups)
+    return q

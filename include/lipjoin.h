@@ -1,0 +1,257 @@
+/*
+ * lipjoin.h -- C-ABI of the MI355X (gfx950) star-join engine.
+ *
+ * Plain C99 types only.  Device pointers are raw HIP device addresses owned by
+ * the caller (the reference's loadToGPU columns); tables, Bloom filters and
+ * scratch are owned by the context.  Every entry point returns an lj_status and
+ * never exits the process; lj_last_error() gives the message of the last
+ * failure on a context.  A context is driven by one host thread at a time.
+ *
+ * Reference interfaces replaced (yliang412/crystal-opt):
+ *   lj_hash_join   <- TimeKeeper hashJoin(...)   src/ops/join.cu:117-148,
+ *                                                 src/ops/bloom_join.cu:121-156
+ *   lj_star_join   <- float runQuery(...)        src/ssb-jo/q33/0-lip.cu:205-302,
+ *                                                 src/ssb-jo/q33/0-naive.cu:178-257
+ *   lj_dim_build   <- build_kernel / build_hashtable_{s,c,d} + their memsets
+ *                     (bloom_join.cu:43-65,128-141; q33/0-lip.cu:114-203,218-257)
+ *   lj_dim_export  <- the device tables those kernels fill (no reference
+ *                     accessor: the reference never copies them back)
+ *   lj_gen_*       <- create_relation_pk / create_relation_fk
+ *                     (src/ops/utils/generator.h:88-136) and loadColumn
+ *                     (src/ssb/ssb_utils.h:161-171), replaced by seeded
+ *                     counter-based generators
+ * The device-side operator API (the crystal/ headers) is mirrored by the headers in
+ * include/lipjoin/block/.
+ */
+#ifndef LIPJOIN_H
+#define LIPJOIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LJ_ABI_VERSION 1
+#define LJ_MAX_DIMS 4
+#define LJ_MAX_PRED_STEPS 4
+
+typedef enum {
+  LJ_OK = 0,
+  LJ_EINVAL = 1,        /* bad descriptor / argument */
+  LJ_ENOMEM = 2,        /* device allocation failed */
+  LJ_EBUDGET = 3,       /* request exceeds the context's HBM budget */
+  LJ_EHIP = 4,          /* HIP runtime error */
+  LJ_EUNSUPPORTED = 5,  /* valid but not implemented combination */
+  LJ_EDOMAIN = 6,       /* keys fell outside the hash domain (reference: UB) */
+  LJ_ETIMEOUT = 7,
+  LJ_ESTATE = 8         /* call not valid in the current state */
+} lj_status;
+
+typedef struct lj_ctx lj_ctx;
+typedef struct lj_dim lj_dim;
+
+/* ---- predicates: crystal/pred.cuh ---------------------------------------- */
+typedef enum { LJ_COMB_SET = 0, LJ_COMB_AND = 1, LJ_COMB_OR = 2 } lj_comb;
+typedef enum { LJ_CMP_LT = 0, LJ_CMP_GT = 1, LJ_CMP_LTE = 2, LJ_CMP_GTE = 3, LJ_CMP_EQ = 4 } lj_cmp;
+
+/* One BlockPred / BlockPredAnd / BlockPredOr call (pred.cuh:35-356).  Steps
+ * run in order; SET overwrites the flag (so GTE followed by SET LTE keeps only
+ * the LTE test, as build_hashtable_d does, q33/0-lip.cu:192-195). */
+typedef struct {
+  int32_t comb;   /* lj_comb */
+  int32_t cmp;    /* lj_cmp */
+  int32_t value;  /* comparison constant */
+} lj_pred_step;
+
+typedef struct {
+  int32_t n_steps;                      /* 0 = InitFlags: every row selected */
+  lj_pred_step steps[LJ_MAX_PRED_STEPS];
+} lj_pred;
+
+/* ---- dimension tables ----------------------------------------------------- */
+typedef enum {
+  LJ_HT_PHT2 = 0  /* crystal PHT_2: (key,val) int32 pairs, slot (key-min)%len,
+                     0 = empty, no key compare (join.cuh:108-189,260-335) */
+} lj_ht_kind;
+
+typedef enum {
+  LJ_BLOOM_NONE = 0,
+  LJ_BLOOM_MOD = 1,   /* HASH1 (bloom.cuh:7): bit (key-min) % (words*32), k = 1 */
+  LJ_BLOOM_MULT = 2   /* multiplicative family, k in 1..3, bits a power of two */
+} lj_bloom_hash;
+
+typedef enum {
+  LJ_BLOOM_STANDARD = 0, /* k independent bit positions over the whole array */
+  LJ_BLOOM_BLOCKED = 1   /* MULT only: all k bits inside one 32-bit word */
+} lj_bloom_layout;
+
+typedef struct {
+  const int32_t* keys;     /* device, n_rows */
+  const int32_t* attr;     /* device, predicate column (may be NULL if pred.n_steps == 0) */
+  const int32_t* payload;  /* device, PHT_2 value column; NULL -> attr */
+  int64_t n_rows;
+  lj_pred pred;
+  int32_t ht_kind;         /* lj_ht_kind */
+  int32_t keys_min;        /* HASH's / HASH1's Z argument */
+  int64_t ht_len;          /* slots (the reference's num_slots) */
+  int32_t bloom_hash;      /* lj_bloom_hash */
+  int32_t bloom_layout;    /* lj_bloom_layout */
+  int32_t bloom_k;         /* hashes per key */
+  int32_t reserved0;
+  int64_t bloom_bits;      /* MOD: rounded up to a multiple of 32 (bloom_join.cu:128);
+                              MULT: power of two in [64, 2^31] */
+} lj_dim_desc;
+
+typedef struct {
+  int64_t n_selected;    /* dimension rows that passed the predicate */
+  int64_t n_oob;         /* selected keys whose hash fell outside the table */
+  int64_t ht_words;      /* int32 words of the PHT (2 * ht_len) */
+  int64_t bloom_words;   /* uint32 words of the Bloom filter (0 if none) */
+  int64_t bloom_bits_set;/* popcount of the Bloom filter */
+  float time_build_ms;   /* device time of the table fill + build launches */
+} lj_dim_info;
+
+/* ---- context -------------------------------------------------------------- */
+/* hip_stream: a hipStream_t to run on (NULL -> the context creates its own
+ * default-priority stream).  hbm_budget_bytes: cap on context-owned device
+ * memory (0 -> no cap). */
+lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* hip_stream, lj_ctx** out);
+void lj_destroy(lj_ctx* ctx);
+const char* lj_last_error(const lj_ctx* ctx);
+const char* lj_status_str(lj_status s);
+int32_t lj_abi_version(void);
+/* sizeof of the public structs, for binding checks: 0 lj_dim_desc, 1 lj_dim_info,
+ * 2 lj_probe_stats, 3 lj_join_desc, 4 lj_join_result, 5 lj_star_desc,
+ * 6 lj_group, 7 lj_star_result; -1 for an unknown index. */
+int64_t lj_abi_struct_size(int32_t which);
+/* Wait for all work queued by the context. */
+lj_status lj_sync(lj_ctx* ctx);
+/* Device memory currently held by the context (tables, filters, scratch). */
+uint64_t lj_bytes_in_use(const lj_ctx* ctx);
+
+/* Build a device-resident dimension (predicate -> Bloom filter -> PHT_2).
+ * The lj_dim stays valid until lj_dim_release. */
+lj_status lj_dim_build(lj_ctx* ctx, const lj_dim_desc* desc, lj_dim** out);
+lj_status lj_dim_get_info(lj_ctx* ctx, const lj_dim* dim, lj_dim_info* info);
+/* Copy the PHT (2*ht_len int32) and/or the Bloom words to host memory; either
+ * pointer may be NULL.  Sizes are checked against the table sizes. */
+lj_status lj_dim_export(lj_ctx* ctx, const lj_dim* dim, int32_t* ht_host, int64_t ht_words,
+                        uint32_t* bloom_host, int64_t bloom_words);
+void lj_dim_release(lj_ctx* ctx, lj_dim* dim);
+
+/* ---- queries ---------------------------------------------------------------- */
+typedef enum {
+  LJ_METHOD_NAIVE = 0,        /* hash probes in join order (q33/{i}-naive.cu) */
+  LJ_METHOD_LIP = 1,          /* all Bloom probes in bloom_order, then hash probes (q33/{i}-lip.cu) */
+  LJ_METHOD_LIP_ADAPTIVE = 2  /* LIP with the Bloom order chosen on the device from sampled pass rates */
+} lj_method;
+
+/* Per-stage statistics reported by the fused probe (exact, all integers).
+ * Stage s is the s-th FK column loaded (join order for NAIVE, Bloom order for
+ * LIP); stage n_dims is the measure column. */
+typedef struct {
+  int64_t rows_in[LJ_MAX_DIMS + 1];   /* rows alive when stage s loads its column */
+  int64_t lines_in[LJ_MAX_DIMS + 1];  /* 32-row (128 B) lines holding >= 1 such row */
+  int64_t rows_bloom_pass;            /* LIP: rows that passed every Bloom filter */
+  int64_t sample_rows;                /* rows handled by the adaptive sampling pass
+                                         (all columns read; not in rows_in/lines_in) */
+  int64_t sample_lines;               /* lines read by that pass, all columns */
+  int32_t order_used[LJ_MAX_DIMS];    /* the order stage 0..n-1 used (adaptive: of workgroup 0) */
+  int64_t n_oob;                      /* FK / group indices outside their domains */
+} lj_probe_stats;
+
+/* Single FK-PK join, optional Bloom pre-filter (join.cu / bloom_join.cu). */
+typedef struct {
+  const int32_t* fact_fk;      /* device */
+  const int32_t* fact_measure; /* device */
+  int64_t n_fact;
+  lj_dim_desc dim;             /* built inside the call (hashJoin semantics) */
+  int32_t method;              /* NAIVE: no Bloom; LIP: Bloom then PHT_2 */
+  int32_t flags;               /* 0 */
+} lj_join_desc;
+
+typedef struct {
+  int64_t n_selected;     /* dimension rows selected */
+  int64_t n_bloom_pass;   /* fact rows that passed the Bloom filter (n_fact for NAIVE) */
+  int64_t n_match;        /* fact rows that matched the PHT */
+  int64_t sum_measure;    /* int64 SUM(measure) over matches */
+  uint64_t sum_product;   /* reference aggregate: sum of (int32)(measure*payload) as u64 (join.cu:96) */
+  float time_build;       /* TimeKeeper fields (ms), join.cu:110-115 */
+  float time_probe;
+  float time_extra;       /* table/filter fills */
+  float time_total;
+  lj_probe_stats stats;
+} lj_join_result;
+
+lj_status lj_hash_join(lj_ctx* ctx, const lj_join_desc* desc, lj_join_result* out);
+
+/* Star join with a dense group-by (Q3.3 shape, q33/{0..5}-{naive,lip}.cu).
+ * group g = (sum_d payload_d * group_mult[d] + group_offset) % n_groups, computed
+ * exactly in 64-bit (q33/0-lip.cu:102-104).  Dimensions come either from
+ * dims[d] (built inside the call: runQuery semantics) or from resident[d]. */
+typedef struct {
+  const int32_t* fact_fk[LJ_MAX_DIMS]; /* device FK column per dimension */
+  const int32_t* fact_measure;          /* device */
+  int64_t n_fact;
+  int32_t n_dims;
+  int32_t method;                       /* lj_method */
+  int32_t order[LJ_MAX_DIMS];           /* join (hash-probe) order, a permutation of 0..n_dims-1 */
+  int32_t bloom_order[LJ_MAX_DIMS];     /* LIP Bloom order; all -1 -> same as order */
+  lj_dim_desc dims[LJ_MAX_DIMS];
+  const lj_dim* resident[LJ_MAX_DIMS];  /* non-NULL -> use this built dimension */
+  int64_t group_mult[LJ_MAX_DIMS];
+  int64_t group_offset;
+  int64_t n_groups;                     /* dense group space, <= 2^26 */
+  int32_t flags;                        /* 0 */
+  int32_t reserved0;
+} lj_star_desc;
+
+typedef struct {
+  int64_t g;                       /* dense group index */
+  int32_t payload[LJ_MAX_DIMS];    /* the dimension payloads of the group (year, c_city, ...) */
+  int64_t sum;                     /* int64 SUM(measure) */
+  int64_t count;                   /* rows in the group */
+  int32_t sum_i32;                 /* reference int32 atomicAdd value (wraps), q33/0-lip.cu:108 */
+  int32_t reserved0;
+} lj_group;
+
+typedef struct {
+  lj_group* groups;        /* caller-provided host array, ascending g */
+  int64_t groups_cap;
+  int64_t n_groups;        /* non-empty groups (may exceed groups_cap: then truncated) */
+  int64_t n_rows;          /* rows aggregated */
+  int64_t sum_total;
+  float time_query_ms;     /* runQuery window: fills + builds + probe + finalize */
+  float time_build_ms;
+  float time_probe_ms;
+  float reserved0;
+  lj_probe_stats stats;
+} lj_star_result;
+
+lj_status lj_star_join(lj_ctx* ctx, const lj_star_desc* desc, lj_star_result* out);
+
+/* ---- seeded synthetic columns ------------------------------------------------ */
+/* Counter-based generator: value j = first + i of stream `stream` depends only
+ * on (seed, stream, j), so host and device produce identical columns. */
+typedef enum {
+  LJ_GEN_UNIFORM = 0,      /* uniform integer in [lo, hi] */
+  LJ_GEN_DATEKEY = 1,      /* yyyymmdd of day lo + U[0, hi), day 0 = 1992-01-01 */
+  LJ_GEN_SEQUENCE = 2,     /* lo + j */
+  LJ_GEN_YEAR_OF_DAY = 3,  /* calendar year of day lo + j */
+  LJ_GEN_DATEKEY_SEQ = 4   /* yyyymmdd of day lo + j */
+} lj_gen_kind;
+
+/* Fill host memory out[0..n) (multithreaded). */
+lj_status lj_gen_host(int32_t kind, uint64_t seed, uint64_t stream, int64_t lo, int64_t hi,
+                      int64_t first, int64_t n, int32_t* out);
+/* Same values, generated on the device into a device buffer. */
+lj_status lj_gen_device(lj_ctx* ctx, int32_t kind, uint64_t seed, uint64_t stream, int64_t lo,
+                        int64_t hi, int64_t first, int64_t n, int32_t* out_dev);
+/* Host Fisher-Yates shuffle of out[0..n) driven by the same generator. */
+lj_status lj_gen_shuffle_host(uint64_t seed, uint64_t stream, int64_t n, int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
