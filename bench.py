@@ -88,8 +88,13 @@ def main():
     cols = W.q33_device_columns(ctx, seed, n)
     torch.cuda.synchronize()
 
+    descs = {}
+
     def run(method, order, bloom_order=None):
-        return ctx.star_join(W.q33_star_desc(cols, n, method, order, bloom_order))
+        key = (method, tuple(order), tuple(bloom_order) if bloom_order else None)
+        if key not in descs:
+            descs[key] = W.q33_star_desc(cols, n, method, order, bloom_order)
+        return ctx.star_join(descs[key])
 
     # ---- per-plan comparison (naive vs LIP in every join order) -----------------
     plans = {}

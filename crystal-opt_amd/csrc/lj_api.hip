@@ -68,8 +68,9 @@ struct lj_ctx {
   uint32_t* touched = nullptr;
   uint32_t touched_cap = 0;
   uint32_t* n_touched = nullptr;
-  GroupOut* out = nullptr;
-  long long* out_hdr = nullptr;
+  void* out_buf = nullptr;       // [kHdrBytes header][GroupOut x touched_cap]
+  GroupOut* out = nullptr;        // inside out_buf
+  long long* out_hdr = nullptr;   // inside out_buf
   long long* stats = nullptr;
   int stats_cap = 0;
   void* host_buf = nullptr;  // pinned: header + first kHostGroups groups
@@ -79,7 +80,9 @@ struct lj_ctx {
   int lds_policy = -1;
 };
 
-static const int kHostGroups = 4096;
+static const int kHostGroups = 4096;   // groups the pinned host buffer holds
+static const int kFirstGroups = 64;    // groups copied back together with the header
+static const size_t kHdrBytes = 256;   // [0] groups hit, [1..16] statistics, [17..24] dimension counters
 static thread_local std::string g_create_err;
 
 static lj_status fail(lj_ctx* c, lj_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
@@ -228,7 +231,7 @@ extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* 
   }
   for (auto& ev : c->ev)
     if (hipEventCreate(&ev) != hipSuccess) return bail(fail(c, LJ_EHIP, "hipEventCreate failed"));
-  if (hipHostMalloc(&c->host_buf, sizeof(long long) * (1 + kStats) + sizeof(GroupOut) * kHostGroups) != hipSuccess)
+  if (hipHostMalloc(&c->host_buf, kHdrBytes + sizeof(GroupOut) * kHostGroups) != hipSuccess)
     return bail(fail(c, LJ_EHIP, "hipHostMalloc failed"));
   c->probe_wg = env_int("LJ_PROBE_WG", 0);
   c->probe_vec = env_int("LJ_PROBE_VEC", 0);
@@ -478,19 +481,20 @@ static lj_status ensure_groups(lj_ctx* c, int64_t n_groups) {
   dev_release(c, c->gprod);
   dev_release(c, c->gpay);
   dev_release(c, c->touched);
-  dev_release(c, c->out);
+  dev_release(c, c->out_buf);
+  c->out_buf = nullptr;
   c->g_cap = 0;
   const int64_t cap = n_groups;
   const uint32_t tcap = (uint32_t)std::min<int64_t>(n_groups, 1 << 20);
   lj_status s;
   if ((s = dev_alloc_t(c, cap, &c->gsum)) != LJ_OK || (s = dev_alloc_t(c, cap, &c->gcount)) != LJ_OK ||
       (s = dev_alloc_t(c, cap, &c->gprod)) != LJ_OK || (s = dev_alloc_t(c, cap * kMaxDims, &c->gpay)) != LJ_OK ||
-      (s = dev_alloc_t(c, (int64_t)tcap, &c->touched)) != LJ_OK || (s = dev_alloc_t(c, (int64_t)tcap, &c->out)) != LJ_OK)
+      (s = dev_alloc_t(c, (int64_t)tcap, &c->touched)) != LJ_OK ||
+      (s = dev_alloc(c, kHdrBytes + sizeof(GroupOut) * tcap, &c->out_buf)) != LJ_OK)
     return s;
-  if (!c->n_touched) {
-    if ((s = dev_alloc_t(c, 64, &c->n_touched)) != LJ_OK) return s;
-    if ((s = dev_alloc_t(c, 1 + kStats, &c->out_hdr)) != LJ_OK) return s;
-  }
+  c->out_hdr = static_cast<long long*>(c->out_buf);
+  c->out = reinterpret_cast<GroupOut*>(static_cast<char*>(c->out_buf) + kHdrBytes);
+  if (!c->n_touched && (s = dev_alloc_t(c, 64, &c->n_touched)) != LJ_OK) return s;
   FillArgs f{};
   add_fill(f, c->gsum, cap * 2);
   add_fill(f, c->gcount, cap * 2);
@@ -583,7 +587,8 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   const bool adaptive = q->method == LJ_METHOD_LIP_ADAPTIVE;
   ProbeConfig cfg;
   cfg.wg = c->probe_wg ? c->probe_wg : 1024;
-  cfg.vec = c->probe_vec ? c->probe_vec : 2;
+  cfg.vec = c->probe_vec ? c->probe_vec : 1;
+  cfg.k0 = S0_GENERIC;
   const int lds_policy = c->lds_policy >= 0 ? c->lds_policy : 0;
   ProbeArgs a{};
   int lds_words = 0;
@@ -614,6 +619,27 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
       P.bloom_lds = lds_words;
       lds_words += (int)w4;
     }
+  }
+  // stage-0 kernel specialisation: the kind of the first filter (every
+  // candidate's kind for the adaptive order; GENERIC when they differ)
+  auto kind_of = [&](int d) -> int {
+    const ProbeDim& P = a.dim[d];
+    const bool l = P.bloom_lds >= 0;
+    switch (P.bh.kind) {
+      case BLOOM_NONE: return S0_NONE;
+      case BLOOM_MOD_POW2: return l ? S0_MODP2_LDS : S0_MODP2_GL;
+      case BLOOM_MOD: return l ? S0_MOD_LDS : S0_MOD_GL;
+      default: return l ? S0_MULT_LDS : S0_MULT_GL;
+    }
+  };
+  if (!lip) {
+    cfg.k0 = S0_PHT;
+  } else if (!adaptive) {
+    cfg.k0 = kind_of(bloom_order[0]);
+  } else {
+    cfg.k0 = kind_of(0);
+    for (int d = 1; d < nd; d++)
+      if (kind_of(d) != cfg.k0) cfg.k0 = S0_GENERIC;
   }
   const size_t lds_bytes = probe_lds_bytes(cfg, lds_words);
   int per_cu = 0;
@@ -686,26 +712,33 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   fa.out_hdr = c->out_hdr;
   fa.out = c->out;
   fa.out_cap = c->touched_cap;
+  for (int d = 0; d < nd; d++) fa.dim_counters[d] = q->resident[d] ? nullptr : dims[d]->counters;
   HIP_TRY(c, launch_finalize(fa, c->stream));
   HIP_TRY(c, hipEventRecord(c->ev[4], c->stream));
 
   // ---- results
+  // one copy: header (groups hit, statistics, dimension counters) and the first groups
   long long* hdr = static_cast<long long*>(c->host_buf);
-  GroupOut* hg = reinterpret_cast<GroupOut*>(hdr + 1 + kStats);
-  HIP_TRY(c, hipMemcpyAsync(hdr, c->out_hdr, sizeof(long long) * (1 + kStats), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipMemcpyAsync(hg, c->out, sizeof(GroupOut) * std::min<size_t>(kHostGroups, c->touched_cap),
-                            hipMemcpyDeviceToHost, c->stream));
-  unsigned long long cnt_h[kMaxDims][2];
-  for (int d = 0; d < nd; d++) {
-    if (!q->resident[d])
-      HIP_TRY(c, hipMemcpyAsync(cnt_h[d], dims[d]->counters, sizeof cnt_h[d], hipMemcpyDeviceToHost, c->stream));
-  }
+  GroupOut* hg = reinterpret_cast<GroupOut*>(static_cast<char*>(c->host_buf) + kHdrBytes);
+  const int64_t first = std::min<int64_t>(kFirstGroups, c->touched_cap);
+  HIP_TRY(c, hipMemcpyAsync(hdr, c->out_buf, kHdrBytes + sizeof(GroupOut) * first, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   const int64_t n_touched = hdr[0];
   memcpy(qo.stats, hdr + 1, sizeof qo.stats);
+  unsigned long long cnt_h[kMaxDims][2];
+  for (int d = 0; d < nd; d++) {
+    cnt_h[d][0] = (unsigned long long)hdr[1 + kStats + 2 * d];
+    cnt_h[d][1] = (unsigned long long)hdr[2 + kStats + 2 * d];
+  }
   if (n_touched > (int64_t)c->touched_cap) {
     cleanup();
     return fail(c, LJ_EUNSUPPORTED, "%lld groups were hit; at most %u can be listed", (long long)n_touched, c->touched_cap);
+  }
+  if (n_touched > first) {
+    const int64_t upto = std::min<int64_t>(n_touched, kHostGroups);
+    HIP_TRY(c, hipMemcpyAsync(hg + first, c->out + first, sizeof(GroupOut) * (size_t)(upto - first),
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
   }
   qo.groups.assign(hg, hg + std::min<int64_t>(n_touched, kHostGroups));
   if (n_touched > kHostGroups) {

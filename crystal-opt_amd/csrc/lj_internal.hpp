@@ -121,9 +121,10 @@ struct FinalizeArgs {
   uint32_t touched_cap;
   int32_t n_blocks;        // probe workgroups whose stats to reduce
   const long long* stats;
-  long long* out_hdr;      // [0] n_touched, [1..kStats] reduced stats
+  long long* out_hdr;      // [0] n_touched, [1..kStats] reduced stats, then 2 counters per dimension
   GroupOut* out;
   int64_t out_cap;
+  const unsigned long long* dim_counters[kMaxDims];  // built-in-call dimensions (or nullptr)
 };
 
 // Host launchers (lj_kernels_*.hip).
@@ -134,11 +135,37 @@ hipError_t launch_gen(int32_t kind, uint64_t key, int64_t lo, int64_t hi, int64_
                       int32_t* out, hipStream_t s);
 hipError_t launch_popcount(const uint32_t* words, int64_t n, unsigned long long* out, hipStream_t s);
 
-// Probe kernel configurations.
+// Probe kernels: one instantiation per (workgroup size, segments per wave
+// tile, stage-0 filter kind).
+enum S0Kind : int {
+  S0_PHT = 0,        // NAIVE: PHT_2 probe
+  S0_NONE = 1,       // LIP dimension without a Bloom filter
+  S0_MODP2_LDS = 2,  // HASH1 over a power-of-two bit count, filter staged in LDS
+  S0_MODP2_GL = 3,   //   ... filter in global memory (L2/MALL)
+  S0_MOD_LDS = 4,    // HASH1, any bit count
+  S0_MOD_GL = 5,
+  S0_MULT_LDS = 6,   // multiplicative family (standard or blocked, k = 1..3)
+  S0_MULT_GL = 7,
+  S0_GENERIC = 8,    // runtime dispatch (adaptive order over mixed filter kinds)
+  S0_NKINDS = 9
+};
+
 struct ProbeConfig {
   int wg;      // threads per workgroup (256, 512, 1024)
-  int vec;     // 256-row segments per wave tile (1, 2, 4)
+  int vec;     // 256-row segments per wave tile (1, 2)
+  int k0;      // S0Kind
 };
+typedef void (*ProbeKernelFn)(ProbeArgs);
+struct ProbeEntry {
+  int wg, vec, k0;
+  const void* fn;
+};
+int probe_entries_pht(const ProbeEntry** out);
+int probe_entries_modp2(const ProbeEntry** out);
+int probe_entries_mod(const ProbeEntry** out);
+int probe_entries_mult(const ProbeEntry** out);
+int probe_entries_generic(const ProbeEntry** out);
+
 size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words);
 hipError_t probe_occupancy(const ProbeConfig& c, size_t lds_bytes, int* blocks_per_cu);
 hipError_t launch_probe(const ProbeConfig& c, const ProbeArgs& a, int grid, size_t lds_bytes, hipStream_t s);

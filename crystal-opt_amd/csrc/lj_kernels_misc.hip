@@ -119,7 +119,17 @@ int64_t build_tile_rows() { return (int64_t)kBuildThreads * kBuildItems; }
 // statistics of the probe.
 __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs a) {
   __shared__ uint32_t n_sh;
+  __shared__ long long part[1024 / kStats][kStats];
   if (threadIdx.x == 0) n_sh = *a.n_touched;
+  // statistics: thread (r, k) sums word k over blocks r, r + 64, ...; then a
+  // column sum over the 64 partial rows
+  {
+    const int k = threadIdx.x % kStats, r = threadIdx.x / kStats;
+    long long acc = 0;
+    if (k != ST_ORDER)
+      for (int b = r; b < a.n_blocks; b += 1024 / kStats) acc += a.stats[(int64_t)b * kStats + k];
+    part[r][k] = acc;
+  }
   __syncthreads();
   const uint32_t n = n_sh;
   const uint32_t listed = n < a.touched_cap ? n : a.touched_cap;
@@ -142,10 +152,14 @@ __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs a) {
     long long acc = 0;
     if (k == ST_ORDER) acc = a.n_blocks > 0 ? a.stats[k] : 0;
     else
-      for (int b = 0; b < a.n_blocks; b++) acc += a.stats[(int64_t)b * kStats + k];
+      for (int r = 0; r < 1024 / kStats; r++) acc += part[r][k];
     a.out_hdr[1 + k] = acc;
   }
   if (threadIdx.x == 0) a.out_hdr[0] = n;
+  if (threadIdx.x < 2 * kMaxDims) {
+    const int d = threadIdx.x / 2;
+    a.out_hdr[1 + kStats + threadIdx.x] = a.dim_counters[d] ? (long long)a.dim_counters[d][threadIdx.x % 2] : 0;
+  }
   __syncthreads();
   if (threadIdx.x == 0) *a.n_touched = 0;
 }

@@ -1,0 +1,853 @@
+// lj_probe_impl.hpp -- the fused fact-table probe of the star join (kernel template).
+//
+// Semantics (what is computed) follow the reference's probe kernels:
+//   NAIVE  q33/{i}-naive.cu:28-91 -- per dimension in join order: (Pred)Load the
+//          FK column, PHT_2 probe (payload), drop misses; then the measure and
+//          the group add.
+//   LIP    q33/{i}-lip.cu:28-112  -- per dimension in Bloom order: (Pred)Load
+//          the FK column, Bloom probe; then PHT_2 probes in join order on the
+//          FKs already loaded; then the measure and the group add.
+//   single join (join.cu:65-108, bloom_join.cu:67-112) is the 1-dimension case
+//          with one group and the extra (int32)(measure*payload) sum.
+// Results are integer sums, so none of the structure below changes them.
+//
+// Structure (how it is computed on CDNA4):
+//   * One kernel per stage-0 filter kind (template K0), so the streaming loop
+//     of a launch holds no per-row dispatch.  Persistent grid; each wave walks
+//     256*VEC-row tiles with a grid stride; lane l owns rows 4l..4l+3 of each
+//     256-row segment and reads them with one 16-byte non-temporal load.  The
+//     next tile's loads are issued before the current tile is processed.
+//   * Stage 0 (the first FK column, read in full) probes its filter for every
+//     row (a Bloom filter staged in LDS, or the PHT for NAIVE).
+//   * A segment with few stage-0 survivors pushes them into a per-wave
+//     64-entry LDS queue (row, key, payload) by ballot/mbcnt; a full queue is
+//     processed with all 64 lanes busy (gather next FK, probe, ..., PHT_2
+//     probes, measure gather, group add).  A segment with many survivors is
+//     processed in place with predicated 16-byte loads.  When a wave sees a
+//     high stage-0 pass rate it also prefetches the next column with the
+//     first one ("eager"), so dense tiles do not wait on a dependent load.
+//   * LIP_ADAPTIVE: every wave first probes ALL Bloom filters on one 256-row
+//     sample segment; the workgroup sums the pass counts in LDS and all its
+//     waves order the filters by ascending pass count.
+//   * Group-by: a 64-slot open-addressing table in LDS per workgroup holds
+//     int64 sums/counts, flushed with 64-bit global atomics at the end; a slot
+//     miss goes straight to the global atomics.  The first global touch of a
+//     group appends it to the `touched` list.
+//   * Exact per-stage statistics: rows alive and 128-byte lines holding an
+//     alive row when each column is read (the algorithmic byte count).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lipjoin/block/crystal.hpp"
+#include "lj_internal.hpp"
+
+namespace lj {
+
+constexpr int kDenseMin = 16;        // stage-0 survivors of a segment above which it is processed in place
+constexpr int32_t kEmptyGroup = -1;
+
+// Wave-uniform counters (32-bit: one wave never sees 2^32 rows); indices into
+// v[] are compile-time constants after unrolling, the measure stage (index
+// n_dims, a runtime value) has its own fields and is placed at flush time.
+struct WaveStats {
+  uint32_t v[kStats];
+  uint32_t meas_rows;
+  uint32_t meas_lines;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int popc_ballot4(uint32_t m) {
+  return __popcll(__ballot(m & 1u)) + __popcll(__ballot(m & 2u)) + __popcll(__ballot(m & 4u)) +
+         __popcll(__ballot(m & 8u));
+}
+
+__device__ __forceinline__ uint32_t mbcnt(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// nonzero bytes of a ballot over "lane holds an alive row" (8 lanes x 4 rows = one 128-byte line)
+__device__ __forceinline__ int lines_of_ballot(unsigned long long b) {
+  b |= b >> 4;
+  b |= b >> 2;
+  b |= b >> 1;
+  return __popcll(b & 0x0101010101010101ull);
+}
+
+// ---- filter probes ----------------------------------------------------------
+__device__ __forceinline__ bool bloom_probe_rt(const ProbeDim& D, const uint32_t* lds, int32_t key) {
+  if (D.bh.kind == BLOOM_NONE) return true;
+  const uint32_t* bf = D.bloom_lds >= 0 ? lds + D.bloom_lds : D.bloom;
+  return bloom_test(D.bh, bf, key);
+}
+
+__device__ __forceinline__ bool pht_probe(const ProbeDim& D, int32_t key, int32_t& pay) {
+  return pht2_probe(D.ht, D.ht_len, D.keys_min, key, pay);
+}
+
+// Stage-0 parameters, hoisted out of the streaming loop.
+struct S0 {
+  const int32_t* fk;
+  const uint32_t* bf;    // staged copy in LDS or the global words
+  const int32_t* ht;
+  BloomHash bh;
+  FastMod fm;
+  int32_t kmin;
+  uint32_t mask;
+};
+
+template <int K0>
+__device__ __forceinline__ bool s0_probe(const S0& s, int32_t key, int32_t& pay);
+
+// Stage 0 over a lane's 4 rows, branch-free: every table word is read
+// unconditionally (a masked bit index is always inside the filter; an invalid
+// PHT slot reads slot 0), so the 4 lookups are in flight together and the
+// result is fixed up with selects.  Returns the alive mask (bit j = row j).
+template <int K0>
+__device__ __forceinline__ uint32_t s0_probe4(const S0& s, const v4i32& k, uint32_t valid4, int32_t (&pay)[4]) {
+  const int32_t kk[4] = {k.x, k.y, k.z, k.w};
+  uint32_t ok = 0;
+  if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_LDS || K0 == S0_MOD_GL) {
+    uint32_t h[4], wv[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t x = (uint32_t)kk[j] - (uint32_t)s.bh.keys_min;
+      if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) h[j] = x & s.mask;
+      else h[j] = fastmod_u32(x, s.bh.mod);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) wv[j] = s.bf[h[j] >> 5];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t bit = (wv[j] >> (h[j] & 31)) & 1u;
+      ok |= (bit & (uint32_t)(kk[j] >= s.bh.keys_min)) << j;
+      pay[j] = 0;
+    }
+  } else if constexpr (K0 == S0_PHT) {
+    uint64_t slot[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int64_t sl = pht_slot(kk[j], s.fm, s.kmin);
+      slot[j] = *reinterpret_cast<const uint64_t*>(s.ht + 2 * (sl < 0 ? 0 : sl));
+      ok |= (uint32_t)(sl >= 0) << j;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      ok &= ~((uint32_t)(slot[j] == 0) << j);
+      pay[j] = (int32_t)(uint32_t)(slot[j] >> 32);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int32_t p = 0;
+      ok |= (uint32_t)s0_probe<K0>(s, kk[j], p) << j;
+      pay[j] = p;
+    }
+  }
+  return ok & valid4;
+}
+
+template <int K0>
+__device__ __forceinline__ bool s0_probe(const S0& s, int32_t key, int32_t& pay) {
+  if constexpr (K0 == S0_PHT) {
+    return pht2_probe(s.ht, s.fm, s.kmin, key, pay);
+  } else if constexpr (K0 == S0_NONE) {
+    return true;
+  } else if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) {
+    const uint32_t x = (uint32_t)key - (uint32_t)s.bh.keys_min;
+    const uint32_t h = x & s.mask;
+    return key >= s.bh.keys_min && ((s.bf[h >> 5] >> (h & 31)) & 1u);
+  } else if constexpr (K0 == S0_MOD_LDS || K0 == S0_MOD_GL) {
+    const uint32_t x = (uint32_t)key - (uint32_t)s.bh.keys_min;
+    const uint32_t h = fastmod_u32(x, s.bh.mod);
+    return key >= s.bh.keys_min && ((s.bf[h >> 5] >> (h & 31)) & 1u);
+  } else if constexpr (K0 == S0_MULT_LDS || K0 == S0_MULT_GL) {
+    return bloom_test(s.bh, s.bf, key);
+  } else {  // S0_GENERIC
+    if (s.bh.kind == BLOOM_NONE) return true;
+    return bloom_test(s.bh, s.bf, key);
+  }
+}
+
+// ---- per-dimension register sets (uniform dimension index; no indexed VGPRs)
+struct Quad {
+  int32_t v[kMaxDims];
+  __device__ __forceinline__ void set(int d, int32_t x) {
+    if (d == 0) v[0] = x;
+    else if (d == 1) v[1] = x;
+    else if (d == 2) v[2] = x;
+    else v[3] = x;
+  }
+  __device__ __forceinline__ int32_t get(int d) const { return d == 0 ? v[0] : d == 1 ? v[1] : d == 2 ? v[2] : v[3]; }
+  __device__ __forceinline__ void clear() { v[0] = v[1] = v[2] = v[3] = 0; }
+};
+
+// ---- group table in LDS --------------------------------------------------
+struct GroupLds {
+  int32_t* key;             // [kGroupSlots]
+  unsigned long long* cnt;  // [kGroupSlots]
+  unsigned long long* sum;  // [kGroupSlots]
+  unsigned long long* prod; // [kGroupSlots]
+  int32_t* pay;             // [kGroupSlots * 4]
+};
+
+__device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, long long s, unsigned long long c,
+                                                 unsigned long long p, const Quad& pay, int nd) {
+  atomicAdd(&a.gsum[g], (unsigned long long)s);
+  if (a.want_product) atomicAdd(&a.gprod[g], p);
+  unsigned long long old = atomicAdd(&a.gcount[g], c);
+  if (old == 0) {
+#pragma unroll
+    for (int d = 0; d < kMaxDims; d++)
+      if (d < nd) a.gpay[g * kMaxDims + d] = pay.v[d];
+    uint32_t idx = atomicAdd(a.n_touched, 1u);
+    if (idx < a.touched_cap) a.touched[idx] = (uint32_t)g;
+  }
+}
+
+// Add one row per alive lane to the workgroup's group table.  Called by every
+// lane of the wave; returns the number of rows (wave-wide) whose group index
+// fell outside [0, n_groups).
+__device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool alive, const Quad& pay, int32_t m) {
+  const int nd = a.n_dims;
+  if (a.n_groups == 1) {
+    // one group (single join): wave-reduce, one LDS atomic per wave
+    long long s = alive ? (long long)m : 0;
+    unsigned long long p = 0;
+    if (a.want_product && alive) p = (unsigned long long)(long long)(int32_t)((uint32_t)m * (uint32_t)pay.v[0]);
+    const unsigned long long b = __ballot(alive);
+    if (b == 0) return 0;
+    s = WaveSum(s);
+    if (a.want_product) p = WaveSum(p);
+    if (lane_id() == 0) {
+      atomicAdd(&gt.sum[0], (unsigned long long)s);
+      atomicAdd(&gt.cnt[0], (unsigned long long)__popcll(b));
+      if (a.want_product) atomicAdd(&gt.prod[0], p);
+      if (gt.key[0] == kEmptyGroup) {
+        gt.key[0] = 0;
+        for (int d = 0; d < kMaxDims; d++) gt.pay[d] = pay.v[d];
+      }
+    }
+    return 0;
+  }
+  int64_t g = 0;
+  if (alive) {
+    int64_t acc = a.goffset;
+#pragma unroll
+    for (int d = 0; d < kMaxDims; d++)
+      if (d < nd) acc += (int64_t)pay.v[d] * a.dim[d].gmult;
+    g = acc % a.n_groups;
+  }
+  const int n_bad = __popcll(__ballot(alive && g < 0));
+  if (!alive || g < 0) return n_bad;
+  const unsigned long long p =
+      a.want_product ? (unsigned long long)(long long)(int32_t)((uint32_t)m * (uint32_t)pay.v[0]) : 0ull;
+  uint32_t slot = ((uint32_t)g * 0x9E3779B1u) >> 26;  // 64 slots
+  for (int probe = 0; probe < kGroupSlots; probe++) {
+    int32_t k = gt.key[slot];
+    if (k == kEmptyGroup) {
+      const int32_t old = atomicCAS(&gt.key[slot], kEmptyGroup, (int32_t)g);
+      if (old == kEmptyGroup) {
+        for (int d = 0; d < kMaxDims; d++) gt.pay[slot * kMaxDims + d] = pay.v[d];
+        k = (int32_t)g;
+      } else {
+        k = old;
+      }
+    }
+    if (k == (int32_t)g) {
+      atomicAdd(&gt.sum[slot], (unsigned long long)(long long)m);
+      atomicAdd(&gt.cnt[slot], 1ull);
+      if (a.want_product) atomicAdd(&gt.prod[slot], p);
+      return n_bad;
+    }
+    slot = (slot + 1) & (kGroupSlots - 1);
+  }
+  global_group_add(a, g, (long long)m, 1ull, p, pay, nd);
+  return n_bad;
+}
+
+// ---- line statistics for a compacted batch (rows ascending with the lane) -----
+__device__ __forceinline__ int batch_lines(bool alive, int32_t row, int32_t& last_line) {
+  // last_line: line of the last alive row of this stage in earlier batches
+  const int32_t line = alive ? (row >> 5) : -1;
+  int32_t pm = line;  // inclusive prefix max over lanes
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t o = __shfl_up(pm, off, 64);
+    if ((int)lane_id() >= off) pm = pm > o ? pm : o;
+  }
+  int32_t prev = __shfl_up(pm, 1, 64);
+  if (lane_id() == 0) prev = -1;
+  if (prev < 0) prev = last_line;
+  const int cnt = __popcll(__ballot(alive && line != prev));
+  const int32_t top = __shfl(pm, 63, 64);
+  if (top >= 0) last_line = top;
+  return cnt;
+}
+
+// ---- per-wave state (named scalars: accessed by compile-time stage numbers) ---
+struct WaveCtx {
+  int nd;
+  int ld0, ld1, ld2, ld3;   // column load order (stage -> dim)
+  int32_t* q_row;
+  int32_t* q_key;
+  int32_t* q_pay;
+  int qcount;
+  int32_t last1, last2, last3, last_meas;
+  __device__ __forceinline__ int ld(int s) const { return s == 0 ? ld0 : s == 1 ? ld1 : s == 2 ? ld2 : ld3; }
+  __device__ __forceinline__ int32_t last(int s) const { return s == 1 ? last1 : s == 2 ? last2 : last3; }
+  __device__ __forceinline__ void set_last(int s, int32_t x) {
+    if (s == 1) last1 = x;
+    else if (s == 2) last2 = x;
+    else last3 = x;
+  }
+};
+
+// Process `cnt` queued stage-0 survivors (lane i takes entry i).
+__device__ __forceinline__ void process_batch(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
+                                              int cnt, WaveStats& st) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int i = (int)lane_id();
+  bool alive = i < cnt;
+  int32_t row = 0;
+  Quad key, pay;
+  key.clear();
+  pay.clear();
+  if (alive) {
+    row = w.q_row[i];
+    key.set(w.ld0, w.q_key[i]);
+    pay.set(w.ld0, w.q_pay[i]);
+  }
+  const int nd = w.nd;
+#pragma unroll
+  for (int s = 1; s < kMaxDims; s++) {
+    if (s >= nd) continue;
+    const int d = w.ld(s);
+    const ProbeDim& D = a.dim[d];
+    st.v[ST_ROWS_IN + s] += __popcll(__ballot(alive));
+    int32_t ls = w.last(s);
+    st.v[ST_LINES_IN + s] += batch_lines(alive, row, ls);
+    w.set_last(s, ls);
+    int32_t k = 0;
+    if (alive) k = D.fk[(uint32_t)row];
+    st.v[ST_OOB] += __popcll(__ballot(alive && k < D.keys_min));
+    if (alive) {
+      key.set(d, k);
+      if (a.method == 0) {
+        int32_t p = 0;
+        alive = pht_probe(D, k, p);
+        pay.set(d, p);
+      } else {
+        alive = bloom_probe_rt(D, lds, k);
+      }
+    }
+  }
+  if (a.method != 0) {
+    st.v[ST_BLOOM_PASS] += __popcll(__ballot(alive));
+#pragma unroll
+    for (int t = 0; t < kMaxDims; t++) {
+      if (t >= nd) continue;
+      const int e = a.order[t];
+      if (alive) {
+        int32_t p = 0;
+        alive = pht_probe(a.dim[e], key.get(e), p);
+        pay.set(e, p);
+      }
+    }
+  }
+  st.meas_rows += __popcll(__ballot(alive));
+  st.meas_lines += batch_lines(alive, row, w.last_meas);
+  int32_t m = 0;
+  if (alive) m = a.measure[(uint32_t)row];
+  st.v[ST_OOB] += group_add(a, gt, alive, pay, m);
+}
+
+// In-place processing of one segment (lane: rows row0..row0+3).  key0 / pay0:
+// stage-0 keys and payloads; mask: alive bits.  If use_eager, `eager` holds
+// the lane's 4 values of the next column already (stage 1's FK column, or the
+// measure column when there is a single dimension).
+__device__ __forceinline__ void process_dense(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
+                                              uint32_t row0, const v4i32& key0, const int32_t (&pay0)[4],
+                                              uint32_t mask, const v4i32& eager, bool use_eager, WaveStats& st) {
+  const int nd = w.nd;
+  const int64_t n = a.n;
+  Quad key[4], pay[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    key[j].clear();
+    pay[j].clear();
+    key[j].set(w.ld0, vget(key0, j));
+    pay[j].set(w.ld0, pay0[j]);
+  }
+#pragma unroll
+  for (int s = 1; s < kMaxDims; s++) {
+    if (s >= nd) continue;
+    const int d = w.ld(s);
+    const ProbeDim& D = a.dim[d];
+    st.v[ST_LINES_IN + s] += lines_of_ballot(__ballot(mask != 0));
+    st.v[ST_ROWS_IN + s] += popc_ballot4(mask);
+    v4i32 kv;
+    if (s == 1 && use_eager) kv = eager;
+    else kv = WavePredLoadVec4(D.fk, row0, n, mask);
+    uint32_t nm = 0, bad = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) bad |= (uint32_t)(((mask >> j) & 1u) && vget(kv, j) < D.keys_min) << j;
+    st.v[ST_OOB] += popc_ballot4(bad);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if ((mask >> j) & 1u) {
+        const int32_t k = vget(kv, j);
+        key[j].set(d, k);
+        bool ok;
+        if (a.method == 0) {
+          int32_t p = 0;
+          ok = pht_probe(D, k, p);
+          pay[j].set(d, p);
+        } else {
+          ok = bloom_probe_rt(D, lds, k);
+        }
+        nm |= (uint32_t)ok << j;
+      }
+    }
+    mask = nm;
+  }
+  if (a.method != 0) {
+    st.v[ST_BLOOM_PASS] += popc_ballot4(mask);
+#pragma unroll
+    for (int t = 0; t < kMaxDims; t++) {
+      if (t >= nd) continue;
+      const int e = a.order[t];
+      uint32_t nm = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if ((mask >> j) & 1u) {
+          int32_t p = 0;
+          const bool ok = pht_probe(a.dim[e], key[j].get(e), p);
+          pay[j].set(e, p);
+          nm |= (uint32_t)ok << j;
+        }
+      }
+      mask = nm;
+    }
+  }
+  st.meas_lines += lines_of_ballot(__ballot(mask != 0));
+  st.meas_rows += popc_ballot4(mask);
+  v4i32 mv;
+  if (nd == 1 && use_eager) mv = eager;
+  else mv = WavePredLoadVec4(a.measure, row0, n, mask);
+#pragma unroll
+  for (int j = 0; j < 4; j++) st.v[ST_OOB] += group_add(a, gt, (mask >> j) & 1u, pay[j], vget(mv, j));
+}
+
+// Append the alive rows of one segment to the wave queue (row order kept).
+__device__ __forceinline__ void enqueue(WaveCtx& w, uint32_t row0, const v4i32& k, const int32_t (&pay)[4],
+                                        uint32_t mask, int cnt_total) {
+  const unsigned long long b0 = __ballot(mask & 1u), b1 = __ballot(mask & 2u), b2 = __ballot(mask & 4u),
+                           b3 = __ballot(mask & 8u);
+  uint32_t pos = (uint32_t)w.qcount + mbcnt(b0) + mbcnt(b1) + mbcnt(b2) + mbcnt(b3);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if ((mask >> j) & 1u) {
+      w.q_row[pos] = (int32_t)(row0 + j);
+      w.q_key[pos] = vget(k, j);
+      w.q_pay[pos] = pay[j];
+      pos++;
+    }
+  }
+  w.qcount += cnt_total;
+}
+
+// One tile of a column into registers: unchecked 16-byte non-temporal loads
+// when the whole tile lies below n (the uniform common case), checked loads
+// for the last, partial tile.
+template <int VEC>
+__device__ __forceinline__ void load_tile(const int32_t* col, uint32_t base, uint32_t lane4, uint32_t n,
+                                          v4i32 (&out)[VEC]) {
+  if (base + (uint32_t)VEC * kSegRows <= n) {
+#pragma unroll
+    for (int v = 0; v < VEC; v++)
+      out[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(col + base + v * kSegRows + lane4));
+  } else {
+#pragma unroll
+    for (int v = 0; v < VEC; v++) out[v] = WaveLoadVec4<true>(col, (int64_t)(base + v * kSegRows + lane4), n);
+  }
+}
+
+struct TileCtx {
+  uint32_t n;        // fact rows
+  uint32_t start;    // first row of the tile space (after the adaptive sample)
+  uint32_t lane4;    // 4 * lane
+  const int32_t* eager_col;
+};
+
+// One tile: stage 0 on the registers `k` (VEC segments of 256 rows), then its
+// survivors.  CHECKED masks rows at or beyond n.  `ke` holds the next column
+// of the same rows when have_eager.  Returns the stage-0 survivor count.
+template <int VEC, int K0, bool CHECKED>
+__device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const uint32_t* lds, GroupLds& gt,
+                                         WaveCtx& w, WaveStats& st, const TileCtx& tc, uint32_t base,
+                                         const v4i32 (&k)[VEC], const v4i32 (&ke)[VEC], bool have_eager, int& oob) {
+  constexpr uint32_t TILE = (uint32_t)VEC * kSegRows;
+  uint32_t mask[VEC];
+  int32_t pay[VEC][4];
+#pragma unroll
+  for (int v = 0; v < VEC; v++) {
+    const uint32_t row0 = base + v * kSegRows + tc.lane4;
+    uint32_t valid4 = 0xFu;
+    if (CHECKED) {
+      valid4 = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) valid4 |= (uint32_t)(row0 + j < tc.n) << j;
+    }
+    const uint32_t ge4 = (uint32_t)(k[v].x >= s0.kmin) | ((uint32_t)(k[v].y >= s0.kmin) << 1) |
+                         ((uint32_t)(k[v].z >= s0.kmin) << 2) | ((uint32_t)(k[v].w >= s0.kmin) << 3);
+    oob += __popc(valid4 & ~ge4);
+    mask[v] = s0_probe4<K0>(s0, k[v], valid4, pay[v]);
+  }
+  const uint32_t nvalid = CHECKED ? (tc.n - base < TILE ? tc.n - base : TILE) : TILE;
+  st.v[ST_ROWS_IN + 0] += nvalid;
+  st.v[ST_LINES_IN + 0] += (nvalid + 31) / 32;
+  int total = 0;
+  // one (not unrolled) pass per segment; the segment's registers are picked
+  // by uniform selects so that no register array is indexed dynamically
+  for (int v = 0; v < VEC; v++) {
+    uint32_t m = mask[0];
+    v4i32 kv = k[0], ev = ke[0];
+    int32_t pv[4] = {pay[0][0], pay[0][1], pay[0][2], pay[0][3]};
+#pragma unroll
+    for (int u = 1; u < VEC; u++) {
+      if (u == v) {
+        m = mask[u];
+        kv = k[u];
+        ev = ke[u];
+#pragma unroll
+        for (int j = 0; j < 4; j++) pv[j] = pay[u][j];
+      }
+    }
+    const int cnt = popc_ballot4(m);
+    total += cnt;
+    if (cnt == 0) continue;
+    const uint32_t row0 = base + v * kSegRows + tc.lane4;
+    if (cnt > kDenseMin || have_eager) {
+      process_dense(a, lds, gt, w, row0, kv, pv, m, ev, have_eager, st);
+      continue;
+    }
+    if (w.qcount + cnt > kQueue) {
+      process_batch(a, lds, gt, w, w.qcount, st);
+      w.qcount = 0;
+    }
+    enqueue(w, row0, kv, pv, m, cnt);
+    if (w.qcount == kQueue) {
+      process_batch(a, lds, gt, w, kQueue, st);
+      w.qcount = 0;
+    }
+  }
+  return total;
+}
+
+// Full tiles t = t0, t0 + stride, ... < n_full.  Every iteration issues the
+// same loads (the next tile, clamped to tile 0 past the end), so the wait for
+// the current tile is a fixed vmcnt and the prefetch stays in flight while the
+// current tile is processed.  EAGER also streams the next column.
+template <int VEC, int K0, bool EAGER>
+__device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, const uint32_t* lds, GroupLds& gt,
+                                            WaveCtx& w, WaveStats& st, const TileCtx& tc, uint32_t t0,
+                                            uint32_t n_full, uint32_t stride, int& oob) {
+  constexpr uint32_t TILE = (uint32_t)VEC * kSegRows;
+  if (t0 >= n_full) return;
+  v4i32 cur[VEC], cure[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; v++) {
+    const uint32_t r = tc.start + t0 * TILE + v * kSegRows + tc.lane4;
+    if (EAGER) cure[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(tc.eager_col + r));
+    else cure[v] = v4i32{0, 0, 0, 0};
+    cur[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(s0.fk + r));
+  }
+  for (uint32_t t = t0; t < n_full; t += stride) {
+    const uint32_t tn = t + stride;
+    const uint32_t tl = tn < n_full ? tn : 0u;
+    v4i32 nxt[VEC], nxte[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; v++) {
+      const uint32_t r = tc.start + tl * TILE + v * kSegRows + tc.lane4;
+      if (EAGER) nxte[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(tc.eager_col + r));
+      else nxte[v] = v4i32{0, 0, 0, 0};
+      nxt[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(s0.fk + r));
+    }
+    // keep the prefetch ahead of the first use of `cur` (the wait below is then
+    // vmcnt(#prefetch loads) instead of vmcnt(0))
+    __builtin_amdgcn_sched_barrier(0);
+    tile_step<VEC, K0, false>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
+#pragma unroll
+    for (int v = 0; v < VEC; v++) {
+      cur[v] = nxt[v];
+      cure[v] = nxte[v];
+    }
+  }
+}
+
+template <int WG, int VEC, int K0>
+__global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
+  // Read the arguments in place from the kernarg segment: indexing the
+  // by-value copy with a (uniform) dimension index would copy it to scratch.
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)args;
+  const ProbeArgs& a = *(const ProbeArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
+#else
+  const ProbeArgs& a = args;
+#endif
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  constexpr int NW = WG / 64;
+  const int wave = threadIdx.x >> 6;
+  const int lane = (int)lane_id();
+
+  uint32_t* lds_filter = reinterpret_cast<uint32_t*>(lds_raw);
+  size_t off = ((size_t)a.lds_filter_words * 4 + 15) & ~(size_t)15;
+  int32_t* q_base = reinterpret_cast<int32_t*>(lds_raw + off) + wave * (3 * kQueue);
+  off += (size_t)NW * 3 * kQueue * 4;
+  GroupLds gt;
+  gt.cnt = reinterpret_cast<unsigned long long*>(lds_raw + off);
+  gt.sum = gt.cnt + kGroupSlots;
+  gt.prod = gt.sum + kGroupSlots;
+  off += 3 * kGroupSlots * 8;
+  gt.key = reinterpret_cast<int32_t*>(lds_raw + off);
+  off += kGroupSlots * 4;
+  gt.pay = reinterpret_cast<int32_t*>(lds_raw + off);
+  off += kGroupSlots * kMaxDims * 4;
+  long long* lds_stats = reinterpret_cast<long long*>(lds_raw + off);
+  off += kStats * 8;
+  unsigned long long* lds_samp = reinterpret_cast<unsigned long long*>(lds_raw + off);  // [kMaxDims + 1]
+
+  // ---- stage the LDS-resident filters, clear the group table
+  for (int d = 0; d < a.n_dims; d++) {
+    const ProbeDim& D = a.dim[d];
+    if (D.bloom_lds >= 0) {
+      const int nw = D.bloom_words;
+      uint32_t* dst = lds_filter + D.bloom_lds;
+      if (((D.bloom_lds | nw) & 3) == 0 && (((uintptr_t)D.bloom) & 15) == 0) {
+        const uint4* src4 = reinterpret_cast<const uint4*>(D.bloom);
+        uint4* dst4 = reinterpret_cast<uint4*>(dst);
+        for (int i = threadIdx.x; i < nw / 4; i += WG) dst4[i] = src4[i];
+      } else {
+        for (int i = threadIdx.x; i < nw; i += WG) dst[i] = D.bloom[i];
+      }
+    }
+  }
+  for (int i = threadIdx.x; i < kGroupSlots; i += WG) {
+    gt.key[i] = kEmptyGroup;
+    gt.cnt[i] = 0;
+    gt.sum[i] = 0;
+    gt.prod[i] = 0;
+  }
+  for (int i = threadIdx.x; i < kStats; i += WG) lds_stats[i] = 0;
+  for (int i = threadIdx.x; i <= kMaxDims; i += WG) lds_samp[i] = 0;
+  __syncthreads();
+
+  WaveStats st;
+#pragma unroll
+  for (int i = 0; i < kStats; i++) st.v[i] = 0;
+  st.meas_rows = st.meas_lines = 0;
+  WaveCtx w;
+  w.nd = a.n_dims;
+  w.q_row = q_base;
+  w.q_key = q_base + kQueue;
+  w.q_pay = q_base + 2 * kQueue;
+  w.qcount = 0;
+  w.last1 = w.last2 = w.last3 = w.last_meas = -1;
+  w.ld0 = a.load_order[0];
+  w.ld1 = a.load_order[1];
+  w.ld2 = a.load_order[2];
+  w.ld3 = a.load_order[3];
+
+  const uint32_t total_waves = gridDim.x * NW;
+  const uint32_t gw = blockIdx.x * NW + wave;
+  const int64_t n64 = a.n;
+  int oob = 0;
+  bool eager = false;  // prefetch the next column with the first one
+
+  // ---- adaptive: sample one segment per wave with every filter --------------
+  if (a.adaptive) {
+    const uint32_t row0 = gw * kSegRows + lane * 4;
+    const bool has = (int64_t)gw * kSegRows < a.sample_rows;
+    v4i32 kk[kMaxDims];
+    uint32_t pass[kMaxDims];
+    uint32_t valid = 0;
+#pragma unroll
+    for (int d = 0; d < kMaxDims; d++) {
+      kk[d] = v4i32{0, 0, 0, 0};
+      pass[d] = 0;
+    }
+    if (has) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) valid |= (uint32_t)((int64_t)row0 + j < n64) << j;
+#pragma unroll
+      for (int d = 0; d < kMaxDims; d++) {
+        if (d >= w.nd) continue;
+        kk[d] = WaveLoadVec4<false>(a.dim[d].fk, row0, n64);
+        uint32_t m = 0, bad = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int32_t key = vget(kk[d], j);
+          const bool vj = (valid >> j) & 1u;
+          bad |= (uint32_t)(vj && key < a.dim[d].keys_min) << j;
+          m |= (uint32_t)(vj && bloom_probe_rt(a.dim[d], lds_filter, key)) << j;
+        }
+        pass[d] = m;
+        st.v[ST_OOB] += popc_ballot4(bad);
+        const int c = popc_ballot4(m);
+        if (lane == 0) atomicAdd(&lds_samp[d], (unsigned long long)c);
+      }
+      const int nv = popc_ballot4(valid);
+      if (lane == 0) atomicAdd(&lds_samp[kMaxDims], (unsigned long long)nv);
+      st.v[ST_SAMPLE_ROWS] += nv;
+      st.v[ST_SAMPLE_LINES] += (long long)w.nd * lines_of_ballot(__ballot(valid != 0));
+    }
+    __syncthreads();
+    const unsigned long long n_sampled = lds_samp[kMaxDims];
+    if (n_sampled > 0) {
+      // rank of each dimension by sampled pass count (ties: lower index first)
+      unsigned long long cntd[kMaxDims];
+#pragma unroll
+      for (int d = 0; d < kMaxDims; d++) cntd[d] = d < w.nd ? lds_samp[d] : ~0ull;
+      int rank[kMaxDims];
+#pragma unroll
+      for (int d = 0; d < kMaxDims; d++) {
+        rank[d] = 0;
+#pragma unroll
+        for (int e = 0; e < kMaxDims; e++) rank[d] += (cntd[e] < cntd[d]) || (cntd[e] == cntd[d] && e < d);
+      }
+      int o[kMaxDims];
+#pragma unroll
+      for (int s2 = 0; s2 < kMaxDims; s2++) {
+        o[s2] = 0;
+#pragma unroll
+        for (int d = 0; d < kMaxDims; d++) o[s2] += rank[d] == s2 ? d : 0;
+      }
+      w.ld0 = uniform(o[0]);
+      w.ld1 = uniform(o[1]);
+      w.ld2 = uniform(o[2]);
+      w.ld3 = uniform(o[3]);
+      // start eager when the chosen first filter passes more than a quarter of the sample
+      const unsigned long long c0 = o[0] == 0 ? cntd[0] : o[0] == 1 ? cntd[1] : o[0] == 2 ? cntd[2] : cntd[3];
+      eager = c0 * 4 > n_sampled;
+    }
+    if (has) {
+      // finish the sampled rows in place: every Bloom filter is already applied
+      uint32_t mask = valid;
+#pragma unroll
+      for (int d = 0; d < kMaxDims; d++)
+        if (d < w.nd) mask &= pass[d];
+      st.v[ST_BLOOM_PASS] += popc_ballot4(mask);
+      Quad pay[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) pay[j].clear();
+#pragma unroll
+      for (int t = 0; t < kMaxDims; t++) {
+        if (t >= w.nd) continue;
+        const int e = a.order[t];
+        const v4i32 ke = e == 0 ? kk[0] : e == 1 ? kk[1] : e == 2 ? kk[2] : kk[3];
+        uint32_t nm = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if ((mask >> j) & 1u) {
+            int32_t p = 0;
+            const bool ok = pht_probe(a.dim[e], vget(ke, j), p);
+            pay[j].set(e, p);
+            nm |= (uint32_t)ok << j;
+          }
+        }
+        mask = nm;
+      }
+      st.v[ST_SAMPLE_LINES] += lines_of_ballot(__ballot(mask != 0));
+      const v4i32 mv = WavePredLoadVec4(a.measure, row0, n64, mask);
+#pragma unroll
+      for (int j = 0; j < 4; j++) st.v[ST_OOB] += group_add(a, gt, (mask >> j) & 1u, pay[j], vget(mv, j));
+    }
+  }
+
+  // ---- streaming loop over tiles -------------------------------------------------
+  const ProbeDim& D0 = a.dim[w.ld0];
+  S0 s0;
+  s0.fk = D0.fk;
+  s0.bf = (K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MULT_LDS || (K0 == S0_GENERIC && D0.bloom_lds >= 0))
+              ? lds_filter + D0.bloom_lds
+              : D0.bloom;
+  s0.ht = D0.ht;
+  s0.bh = D0.bh;
+  s0.fm = D0.ht_len;
+  s0.kmin = D0.keys_min;
+  s0.mask = D0.bh.mod.pow2_mask;
+  const int32_t* eager_col = w.nd > 1 ? a.dim[w.ld1].fk : a.measure;
+
+  TileCtx tc;
+  tc.n = (uint32_t)n64;
+  tc.start = (uint32_t)a.sample_rows;
+  tc.lane4 = (uint32_t)lane * 4;
+  tc.eager_col = eager_col;
+  constexpr uint32_t TILE = (uint32_t)VEC * kSegRows;
+  const uint32_t rest = tc.n > tc.start ? tc.n - tc.start : 0u;
+  const uint32_t n_full = rest / TILE;
+  const uint32_t n_tiles = (rest + TILE - 1) / TILE;
+  // The wave's first tile and the table's last, partial tile go through the
+  // checked path; the first tile's pass rate (or the adaptive sample) decides
+  // whether the full-tile loop prefetches the next column too.
+  const uint32_t tail_tile = (n_full < n_tiles && n_full >= total_waves && gw == n_full % total_waves)
+                                 ? n_full : 0xFFFFFFFFu;
+  for (int pass = 0; pass < 2; pass++) {
+    const uint32_t tt = pass == 0 ? gw : tail_tile;
+    if (tt < n_tiles) {
+      const uint32_t base = tc.start + tt * TILE;
+      v4i32 k[VEC], ke[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; v++) {
+        k[v] = WaveLoadVec4<true>(s0.fk, (int64_t)(base + v * kSegRows + tc.lane4), n64);
+        ke[v] = k[v];
+      }
+      const int tot = tile_step<VEC, K0, true>(a, s0, lds_filter, gt, w, st, tc, base, k, ke, false, oob);
+      const uint32_t nv = tc.n - base < TILE ? tc.n - base : TILE;
+      if (pass == 0 && !a.adaptive) eager = (uint32_t)tot * 4 > nv;
+    }
+    if (pass == 0) {
+      if (eager) stream_loop<VEC, K0, true>(a, s0, lds_filter, gt, w, st, tc, gw + total_waves, n_full, total_waves, oob);
+      else stream_loop<VEC, K0, false>(a, s0, lds_filter, gt, w, st, tc, gw + total_waves, n_full, total_waves, oob);
+    }
+  }
+  if (w.qcount > 0) process_batch(a, lds_filter, gt, w, w.qcount, st);
+
+  // ---- flush: statistics and groups ----------------------------------------------
+  {
+    int o = oob;
+#pragma unroll
+    for (int off2 = 32; off2 > 0; off2 >>= 1) o += __shfl_down(o, off2, 64);
+    if (lane == 0) {
+      st.v[ST_OOB] += o;
+      for (int i = 0; i < kStats; i++)
+        if (i != ST_ORDER && st.v[i]) atomicAdd((unsigned long long*)&lds_stats[i], (unsigned long long)st.v[i]);
+      atomicAdd((unsigned long long*)&lds_stats[ST_ROWS_IN + a.n_dims], (unsigned long long)st.meas_rows);
+      atomicAdd((unsigned long long*)&lds_stats[ST_LINES_IN + a.n_dims], (unsigned long long)st.meas_lines);
+      if (blockIdx.x == 0 && wave == 0)
+        lds_stats[ST_ORDER] = (long long)(w.ld0 & 0xff) | ((long long)(w.ld1 & 0xff) << 8) |
+                              ((long long)(w.ld2 & 0xff) << 16) | ((long long)(w.ld3 & 0xff) << 24);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kGroupSlots; i += WG) {
+    const int32_t g = gt.key[i];
+    if (g != kEmptyGroup && gt.cnt[i] != 0) {
+      Quad pay;
+      for (int d = 0; d < kMaxDims; d++) pay.v[d] = gt.pay[i * kMaxDims + d];
+      global_group_add(a, g, (long long)gt.sum[i], gt.cnt[i], gt.prod[i], pay, a.n_dims);
+    }
+  }
+  for (int i = threadIdx.x; i < kStats; i += WG) a.stats[(int64_t)blockIdx.x * kStats + i] = lds_stats[i];
+}
+
+// Host-side table of instantiated kernels, filled by the instantiation units.
+#define LJ_PROBE_SHAPES(X, K) X(256, 1, K) X(256, 2, K) X(512, 1, K) X(512, 2, K) X(1024, 1, K) X(1024, 2, K)
+
+}  // namespace lj

@@ -1,0 +1,14 @@
+// Fused probe kernels for stage-0 kinds: S0_MOD_LDS S0_MOD_GL.
+#include "lj_probe_impl.hpp"
+
+namespace lj {
+#define LJ_ENTRY(W, V, K) {W, V, K, reinterpret_cast<const void*>(&probe_kernel<W, V, K>)},
+static const ProbeEntry k_entries[] = {
+    LJ_PROBE_SHAPES(LJ_ENTRY, S0_MOD_LDS)
+    LJ_PROBE_SHAPES(LJ_ENTRY, S0_MOD_GL)
+};
+int probe_entries_mod(const ProbeEntry** out) {
+  *out = k_entries;
+  return (int)(sizeof(k_entries) / sizeof(k_entries[0]));
+}
+}  // namespace lj

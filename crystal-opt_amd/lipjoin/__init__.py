@@ -190,8 +190,12 @@ class Context:
         return out
 
     # -- star join (runQuery) ---------------------------------------------------------
-    def star_join(self, desc, groups_cap=1 << 16, allow_domain=False):
-        arr = (_abi.Group * groups_cap)()
+    def star_join(self, desc, groups_cap=4096, allow_domain=False):
+        # the host result buffer is kept and reused across calls
+        if getattr(self, "_gbuf_cap", 0) < groups_cap:
+            self._gbuf = (_abi.Group * groups_cap)()
+            self._gbuf_cap = groups_cap
+        arr = self._gbuf
         r = _abi.StarResult()
         r.groups, r.groups_cap = C.cast(arr, C.POINTER(_abi.Group)), groups_cap
         st = self._check(lib().lj_star_join(self.h, C.byref(desc), C.byref(r)),
