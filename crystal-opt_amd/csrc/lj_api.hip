@@ -3,11 +3,13 @@
 // Orchestration mirrors the reference drivers' runQuery / hashJoin
 // (src/ssb-jo/q33/0-lip.cu:205-302, src/ops/bloom_join.cu:121-156): allocate and
 // clear the tables, build every dimension, run the fused probe, collect the
-// groups.  Differences by design: tables come from a per-context caching
-// allocator (released back to it, never leaked), fills are kernels (so a query
-// can be graph-captured), all dimensions are built by one launch, the group
-// arrays are kept zero by the finalize kernel instead of a 7 MB memset per
-// query, and every failure is a status code.
+// groups.  Differences by design: a query is two kernel launches (one build
+// of all its dimensions; the probe, whose last workgroup collects the groups)
+// and one small copy back; tables come from a per-context caching allocator and
+// are returned to it (never leaked) and cleared there, behind the query, so
+// the next query starts on zeroed tables; the dense group arrays are kept zero
+// by the result collection instead of a 7 MB memset per query; every failure
+// is a status code.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,10 +28,6 @@
 #include "lj_internal.hpp"
 
 using namespace lj;
-
-namespace lj {
-int64_t build_tile_rows();
-}
 
 // ------------------------------------------------------------------ context --
 struct lj_dim {
@@ -55,6 +53,7 @@ struct lj_ctx {
   uint64_t cached = 0;  // bytes parked in the free lists
   std::string err;
   std::multimap<size_t, void*> free_blocks;
+  std::multimap<size_t, void*> zero_blocks;  // free blocks known to be all zero (cleared on release)
   std::unordered_map<void*, size_t> live;
   int n_cu = 0;
   int max_lds = 65536;
@@ -67,7 +66,7 @@ struct lj_ctx {
   int32_t* gpay = nullptr;
   uint32_t* touched = nullptr;
   uint32_t touched_cap = 0;
-  uint32_t* n_touched = nullptr;
+  uint32_t* n_touched = nullptr;  // [0] groups hit; [32] workgroups finished (probe kernel)
   void* out_buf = nullptr;       // [kHdrBytes header][GroupOut x touched_cap]
   GroupOut* out = nullptr;        // inside out_buf
   long long* out_hdr = nullptr;   // inside out_buf
@@ -78,11 +77,14 @@ struct lj_ctx {
   int probe_wg = 0;
   int probe_vec = 0;
   int lds_policy = -1;
+  double clock_khz = 100000.0;  // device wall clock (kernel phase timestamps)
+  std::map<std::pair<int, size_t>, int> occupancy;  // (config key, LDS bytes) -> workgroups per CU
 };
 
 static const int kHostGroups = 4096;   // groups the pinned host buffer holds
 static const int kFirstGroups = 64;    // groups copied back together with the header
-static const size_t kHdrBytes = 256;   // [0] groups hit, [1..16] statistics, [17..24] dimension counters
+static const size_t kHdrBytes = 256;   // [0] groups hit, [1..16] statistics, [17..24] dimension counters,
+                                       // [25..27] kernel phase clocks
 static thread_local std::string g_create_err;
 
 static lj_status fail(lj_ctx* c, lj_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
@@ -154,6 +156,57 @@ static void dev_release(lj_ctx* c, void* p) {
   c->live.erase(it);
 }
 
+// Blocks whose every byte is zero.  A released table is cleared by a fill
+// queued on the context's stream behind the work that used it, so the next
+// query finds it clean without a memset on its own critical path.
+static lj_status dev_alloc_zero(lj_ctx* c, size_t bytes, void** out) {
+  const size_t sz = round_block(bytes ? bytes : 1);
+  auto it = c->zero_blocks.find(sz);
+  if (it != c->zero_blocks.end()) {
+    *out = it->second;
+    c->zero_blocks.erase(it);
+    c->cached -= sz;
+    c->live[*out] = sz;
+    c->in_use += sz;
+    return LJ_OK;
+  }
+  lj_status s = dev_alloc(c, sz, out);
+  if (s != LJ_OK) return s;
+  FillArgs f{};
+  f.r[0].ptr = static_cast<uint32_t*>(*out);
+  f.r[0].words = (int64_t)(sz / 4);
+  f.n = 1;
+  HIP_TRY(c, launch_fill(f, c->stream));
+  return LJ_OK;
+}
+
+// Queue the clear of a block and park it in the zero pool.
+static void dev_release_zero(lj_ctx* c, void* p, FillArgs& f) {
+  if (!p) return;
+  auto it = c->live.find(p);
+  if (it == c->live.end()) return;
+  if (f.n == kMaxFill) {
+    (void)launch_fill(f, c->stream);
+    f.n = 0;
+  }
+  f.r[f.n].ptr = static_cast<uint32_t*>(p);
+  f.r[f.n].words = (int64_t)(it->second / 4);
+  f.r[f.n].value = 0;
+  f.n++;
+  c->zero_blocks.emplace(it->second, p);
+  c->cached += it->second;
+  c->in_use -= it->second;
+  c->live.erase(it);
+}
+
+template <typename T>
+static lj_status dev_alloc_zero_t(lj_ctx* c, int64_t count, T** out) {
+  void* p = nullptr;
+  lj_status s = dev_alloc_zero(c, (size_t)count * sizeof(T), &p);
+  *out = static_cast<T*>(p);
+  return s;
+}
+
 template <typename T>
 static lj_status dev_alloc_t(lj_ctx* c, int64_t count, T** out) {
   void* p = nullptr;
@@ -221,6 +274,11 @@ extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* 
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail(c, LJ_EHIP, "hipGetDeviceProperties failed"));
   c->n_cu = prop.multiProcessorCount;
+  {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+      c->clock_khz = khz;
+  }
   c->max_lds = (int)prop.sharedMemPerBlock;
   if (hip_stream) {
     c->stream = static_cast<hipStream_t>(hip_stream);
@@ -246,6 +304,7 @@ extern "C" void lj_destroy(lj_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->live) (void)hipFree(kv.first);
   for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
+  for (auto& kv : c->zero_blocks) (void)hipFree(kv.second);
   for (auto& ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->host_buf) (void)hipHostFree(c->host_buf);
@@ -329,11 +388,15 @@ static void dim_hashes(lj_dim* D) {
   }
 }
 
+// Return a dimension's tables to the zero pool; the clears are queued on the
+// stream (after whatever already used the tables).
 static void dim_free(lj_ctx* c, lj_dim* D) {
   if (!D) return;
-  dev_release(c, D->ht);
-  dev_release(c, D->bloom);
-  dev_release(c, D->counters);
+  FillArgs f{};
+  dev_release_zero(c, D->ht, f);
+  dev_release_zero(c, D->bloom, f);
+  dev_release_zero(c, D->counters, f);
+  if (f.n) (void)launch_fill(f, c->stream);
   delete D;
 }
 
@@ -345,9 +408,9 @@ static lj_status dim_alloc(lj_ctx* c, const lj_dim_desc* desc, lj_dim** out) {
   D->desc = *desc;
   if (!D->desc.payload) D->desc.payload = D->desc.attr;
   dim_hashes(D);
-  if ((s = dev_alloc_t(c, D->ht_words, &D->ht)) != LJ_OK ||
-      (D->bloom_words && (s = dev_alloc_t(c, D->bloom_words, &D->bloom)) != LJ_OK) ||
-      (s = dev_alloc_t(c, 4, &D->counters)) != LJ_OK) {
+  if ((s = dev_alloc_zero_t(c, D->ht_words, &D->ht)) != LJ_OK ||
+      (D->bloom_words && (s = dev_alloc_zero_t(c, D->bloom_words, &D->bloom)) != LJ_OK) ||
+      (s = dev_alloc_zero_t(c, 4, &D->counters)) != LJ_OK) {
     dim_free(c, D);
     return s;
   }
@@ -364,42 +427,31 @@ static void add_fill(FillArgs& f, void* p, int64_t words) {
 }
 
 // Queue fill + build of up to 4 allocated dimensions.
-static lj_status dims_launch(lj_ctx* c, lj_dim* const* dims, int n, hipEvent_t after_fill) {
-  FillArgs f{};
-  BuildArgs b{};
-  int64_t blocks = 0;
-  const int64_t tile = build_tile_rows();
-  for (int i = 0; i < n; i++) {
-    lj_dim* D = dims[i];
-    add_fill(f, D->ht, D->ht_words);
-    add_fill(f, D->bloom, D->bloom_words);
-    add_fill(f, D->counters, 8);
-    BuildDim& bd = b.dim[b.n_dims];
-    bd.keys = D->desc.keys;
-    bd.attr = D->desc.attr;
-    bd.payload = D->desc.payload;
-    bd.n = D->desc.n_rows;
-    bd.pred.n_steps = D->desc.pred.n_steps;
-    for (int k = 0; k < 4; k++) {
-      bd.pred.comb[k] = D->desc.pred.steps[k].comb;
-      bd.pred.cmp[k] = D->desc.pred.steps[k].cmp;
-      bd.pred.value[k] = D->desc.pred.steps[k].value;
-    }
-    bd.ht = D->ht;
-    bd.ht_len = D->ht_len;
-    bd.keys_min = D->desc.keys_min;
-    bd.bloom = D->bloom;
-    bd.bh = D->bh;
-    bd.counters = D->counters;
-    bd.block_begin = blocks;
-    blocks += (D->desc.n_rows + tile - 1) / tile;
-    b.n_dims++;
+static void set_build_dim(const lj_dim* D, BuildDim& bd) {
+  bd.keys = D->desc.keys;
+  bd.attr = D->desc.attr;
+  bd.payload = D->desc.payload;
+  bd.n = D->desc.n_rows;
+  bd.pred.n_steps = D->desc.pred.n_steps;
+  for (int k = 0; k < 4; k++) {
+    bd.pred.comb[k] = D->desc.pred.steps[k].comb;
+    bd.pred.cmp[k] = D->desc.pred.steps[k].cmp;
+    bd.pred.value[k] = D->desc.pred.steps[k].value;
   }
-  HIP_TRY(c, launch_fill(f, c->stream));
-  if (after_fill) HIP_TRY(c, hipEventRecord(after_fill, c->stream));
-  // workgroup w builds the last dimension whose block_begin <= w (an empty
-  // dimension owns an empty range)
-  if (blocks > 0) HIP_TRY(c, launch_build(b, blocks, c->stream));
+  bd.ht = D->ht;
+  bd.ht_len = D->ht_len;
+  bd.keys_min = D->desc.keys_min;
+  bd.bloom = D->bloom;
+  bd.bh = D->bh;
+  bd.counters = D->counters;
+}
+
+// Build of allocated (all-zero) dimensions: one launch for all of them.
+static lj_status dims_launch(lj_ctx* c, lj_dim* const* dims, int n, long long* clock) {
+  BuildArgs b{};
+  for (int i = 0; i < n; i++) set_build_dim(dims[i], b.dim[b.n_dims++]);
+  b.clock = clock;
+  HIP_TRY(c, launch_build(b, c->n_cu, c->stream));
   return LJ_OK;
 }
 
@@ -494,7 +546,7 @@ static lj_status ensure_groups(lj_ctx* c, int64_t n_groups) {
     return s;
   c->out_hdr = static_cast<long long*>(c->out_buf);
   c->out = reinterpret_cast<GroupOut*>(static_cast<char*>(c->out_buf) + kHdrBytes);
-  if (!c->n_touched && (s = dev_alloc_t(c, 64, &c->n_touched)) != LJ_OK) return s;
+  if (!c->n_touched && (s = dev_alloc_t(c, 64, &c->n_touched)) != LJ_OK) return s;  // [0] touched, [32] done
   FillArgs f{};
   add_fill(f, c->gsum, cap * 2);
   add_fill(f, c->gcount, cap * 2);
@@ -643,7 +695,16 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   }
   const size_t lds_bytes = probe_lds_bytes(cfg, lds_words);
   int per_cu = 0;
-  HIP_TRY(c, probe_occupancy(cfg, lds_bytes, &per_cu));
+  {
+    const auto key = std::make_pair(cfg.wg * 1000 + cfg.vec * 100 + cfg.k0, lds_bytes);
+    auto it = c->occupancy.find(key);
+    if (it == c->occupancy.end()) {
+      HIP_TRY(c, probe_occupancy(cfg, lds_bytes, &per_cu));
+      c->occupancy[key] = per_cu;
+    } else {
+      per_cu = it->second;
+    }
+  }
   if (per_cu < 1) {
     cleanup();
     return fail(c, LJ_EUNSUPPORTED, "probe configuration wg=%d vec=%d with %zu B LDS does not fit", cfg.wg, cfg.vec, lds_bytes);
@@ -686,35 +747,24 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   a.touched_cap = c->touched_cap;
   a.stats = c->stats;
 
-  // ---- queue: fill + build, probe, finalize
+  a.done = c->n_touched + 32;
+  a.out_hdr = c->out_hdr;
+  a.out = c->out;
+  a.out_cap = c->touched_cap;
+  for (int d = 0; d < nd; d++) a.dim_counters[d] = q->resident[d] ? nullptr : dims[d]->counters;
+
+  // ---- two launches: the build of the query's own dimensions, then the probe
+  // (which also collects the result groups in its last workgroup)
+  const bool built_here = n_owned > 0;
   HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
-  if (n_owned > 0) {
-    if ((s = dims_launch(c, owned, n_owned, c->ev[1])) != LJ_OK) {
+  if (built_here) {
+    if ((s = dims_launch(c, owned, n_owned, c->out_hdr + kClockSlot)) != LJ_OK) {
       cleanup();
       return s;
     }
-  } else {
-    HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
   }
-  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
-  if (q->n_fact > 0) HIP_TRY(c, launch_probe(cfg, a, (int)grid, lds_bytes, c->stream));
-  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
-  FinalizeArgs fa{};
-  fa.gsum = c->gsum;
-  fa.gcount = c->gcount;
-  fa.gprod = c->gprod;
-  fa.gpay = c->gpay;
-  fa.touched = c->touched;
-  fa.n_touched = c->n_touched;
-  fa.touched_cap = c->touched_cap;
-  fa.n_blocks = q->n_fact > 0 ? (int)grid : 0;
-  fa.stats = c->stats;
-  fa.out_hdr = c->out_hdr;
-  fa.out = c->out;
-  fa.out_cap = c->touched_cap;
-  for (int d = 0; d < nd; d++) fa.dim_counters[d] = q->resident[d] ? nullptr : dims[d]->counters;
-  HIP_TRY(c, launch_finalize(fa, c->stream));
-  HIP_TRY(c, hipEventRecord(c->ev[4], c->stream));
+  HIP_TRY(c, launch_probe(cfg, a, (int)grid, lds_bytes, c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
 
   // ---- results
   // one copy: header (groups hit, statistics, dimension counters) and the first groups
@@ -722,7 +772,11 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   GroupOut* hg = reinterpret_cast<GroupOut*>(static_cast<char*>(c->host_buf) + kHdrBytes);
   const int64_t first = std::min<int64_t>(kFirstGroups, c->touched_cap);
   HIP_TRY(c, hipMemcpyAsync(hdr, c->out_buf, kHdrBytes + sizeof(GroupOut) * first, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
+  // the built tables go back to the zero pool now: their clears queue behind the copy
+  cleanup();
+  n_owned = 0;
+  HIP_TRY(c, hipEventSynchronize(c->ev[2]));
   const int64_t n_touched = hdr[0];
   memcpy(qo.stats, hdr + 1, sizeof qo.stats);
   unsigned long long cnt_h[kMaxDims][2];
@@ -747,10 +801,15 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
                          sizeof(GroupOut) * (size_t)(n_touched - kHostGroups), hipMemcpyDeviceToHost));
   }
   std::sort(qo.groups.begin(), qo.groups.end(), [](const GroupOut& x, const GroupOut& y) { return x.g < y.g; });
-  HIP_TRY(c, hipEventElapsedTime(&qo.t_fill, c->ev[0], c->ev[1]));
-  HIP_TRY(c, hipEventElapsedTime(&qo.t_build, c->ev[1], c->ev[2]));
-  HIP_TRY(c, hipEventElapsedTime(&qo.t_probe, c->ev[2], c->ev[3]));  // the fused probe kernel alone
-  HIP_TRY(c, hipEventElapsedTime(&qo.t_total, c->ev[0], c->ev[4]));
+  // phases from the kernels' own wall-clock stamps: build start, probe start, probe end
+  {
+    const double to_ms = 1.0 / c->clock_khz;
+    const long long t0 = hdr[kClockSlot], t1 = hdr[kClockSlot + 1], t2 = hdr[kClockSlot + 2];
+    qo.t_fill = 0.f;
+    qo.t_build = built_here ? (float)((t1 - t0) * to_ms) : 0.f;
+    qo.t_probe = (float)((t2 - t1) * to_ms);
+  }
+  HIP_TRY(c, hipEventElapsedTime(&qo.t_total, c->ev[0], c->ev[1]));
   int64_t build_oob = 0;
   for (int d = 0; d < nd; d++) {
     if (q->resident[d]) {

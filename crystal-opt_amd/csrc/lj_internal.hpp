@@ -15,6 +15,9 @@ constexpr int kGroupSlots = 64;      // LDS-privatised groups per workgroup
 constexpr int kQueue = 64;           // survivor queue entries per wave
 constexpr int kStats = 16;           // per-workgroup statistics words
 constexpr int kSegRows = 256;        // rows of one wave segment (64 lanes x 4)
+// result header words: [0] groups hit, [1..kStats] statistics, then 2 counters
+// per dimension, then 3 kernel wall-clock stamps (start, build done, end)
+constexpr int kClockSlot = 1 + kStats + 2 * kMaxDims;
 
 // statistics word indices
 enum StatIdx : int {
@@ -25,6 +28,23 @@ enum StatIdx : int {
   ST_SAMPLE_LINES = 12,
   ST_OOB = 13,
   ST_ORDER = 14          // order of workgroup 0, 8 bits per stage
+};
+
+struct GroupOut;
+
+struct BuildDim {
+  const int32_t* keys;
+  const int32_t* attr;
+  const int32_t* payload;
+  int64_t n;
+  PredProgram pred;
+  int32_t* ht;
+  FastMod ht_len;
+  int32_t keys_min;
+  int32_t pad0;
+  uint32_t* bloom;         // nullptr: no filter
+  BloomHash bh;
+  unsigned long long* counters;  // [0] selected rows, [1] keys outside the domain
 };
 
 struct ProbeDim {
@@ -64,28 +84,21 @@ struct ProbeArgs {
   uint32_t touched_cap;
   int32_t pad1;
   long long* stats;        // [gridDim.x * kStats]
+  // result: the last workgroup to finish copies the groups out and resets the
+  // dense accumulators
+  uint32_t* done;          // workgroups finished (reset to 0 by the last one)
+  long long* out_hdr;      // [0] groups hit, [1..kStats] statistics, then 2 counters per dimension
+  GroupOut* out;
+  int64_t out_cap;
+  const unsigned long long* dim_counters[kMaxDims];
 };
 
-struct BuildDim {
-  const int32_t* keys;
-  const int32_t* attr;
-  const int32_t* payload;
-  int64_t n;
-  PredProgram pred;
-  int32_t* ht;
-  FastMod ht_len;
-  int32_t keys_min;
-  int32_t pad0;
-  uint32_t* bloom;         // nullptr: no filter
-  BloomHash bh;
-  unsigned long long* counters;  // [0] selected rows, [1] keys outside the domain
-  int64_t block_begin;
-};
 
 struct BuildArgs {
   BuildDim dim[kMaxDims];
   int32_t n_dims;
   int32_t pad0;
+  long long* clock;        // if set: wall-clock stamp at the start of the build
 };
 
 struct FillRegion {
@@ -111,26 +124,10 @@ struct GroupOut {
   int32_t pay[kMaxDims];
 };
 
-struct FinalizeArgs {
-  unsigned long long* gsum;
-  unsigned long long* gcount;
-  unsigned long long* gprod;
-  const int32_t* gpay;
-  const uint32_t* touched;
-  uint32_t* n_touched;
-  uint32_t touched_cap;
-  int32_t n_blocks;        // probe workgroups whose stats to reduce
-  const long long* stats;
-  long long* out_hdr;      // [0] n_touched, [1..kStats] reduced stats, then 2 counters per dimension
-  GroupOut* out;
-  int64_t out_cap;
-  const unsigned long long* dim_counters[kMaxDims];  // built-in-call dimensions (or nullptr)
-};
 
 // Host launchers (lj_kernels_*.hip).
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
-hipError_t launch_build(const BuildArgs& a, int64_t n_blocks, hipStream_t s);
-hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s);
+hipError_t launch_build(const BuildArgs& a, int n_cu, hipStream_t s);
 hipError_t launch_gen(int32_t kind, uint64_t key, int64_t lo, int64_t hi, int64_t first, int64_t n,
                       int32_t* out, hipStream_t s);
 hipError_t launch_popcount(const uint32_t* words, int64_t n, unsigned long long* out, hipStream_t s);

@@ -1,11 +1,11 @@
-// lj_kernels_misc.hip -- table fills, dimension build, result finalize,
-// synthetic column generation.
+// lj_kernels_misc.hip -- table fills, dimension build, synthetic column
+// generation, Bloom popcount.
 //
 // The dimension build follows build_kernel (src/ops/bloom_join.cu:43-65) and
-// build_hashtable_{s,c,d} (src/ssb-jo/q33/0-lip.cu:114-203): load a tile of
-// (key, attr), evaluate the predicate, insert selected keys into the Bloom
-// filter and into PHT_2 (key -> payload).  All dimensions of one query are
-// built by a single launch whose workgroups are partitioned by dimension.
+// build_hashtable_{s,c,d} (src/ssb-jo/q33/0-lip.cu:114-203): evaluate the
+// predicate on (key, attr), insert selected keys into the Bloom filter and
+// into PHT_2 (key -> payload).  All dimensions of one query are built by a
+// single launch.
 #include <hip/hip_runtime.h>
 
 #include "lipjoin/block/crystal.hpp"
@@ -49,123 +49,61 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
 }
 
 // --------------------------------------------------------------- build ----
+// Predicate, Bloom insert, PHT_2 insert (CAS of the key, then the
+// unconditional value store) -- build_kernel of src/ops/bloom_join.cu:43-65 and
+// build_hashtable_{s,c,d} of src/ssb-jo/q33/0-lip.cu:114-203.  Every dimension
+// of the query in one launch: grid-stride over each dimension's rows in turn;
+// the selected / out-of-domain counts reduce per workgroup to one atomic each.
 constexpr int kBuildThreads = 256;
-constexpr int kBuildItems = 4;
 
 __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
-  constexpr int BLOCK_THREADS = kBuildThreads;
-  constexpr int ITEMS_PER_THREAD = kBuildItems;
-  __shared__ unsigned long long red[BLOCK_THREADS / 64];
-
-  int d = 0;
-  for (int i = 1; i < a.n_dims; i++)
-    if ((int64_t)blockIdx.x >= a.dim[i].block_begin) d = i;
-  const BuildDim& D = a.dim[d];
-
-  const int64_t tile_offset = ((int64_t)blockIdx.x - D.block_begin) * TILE_SIZE;
-  int num_tile_items = TILE_SIZE;
-  if (tile_offset + TILE_SIZE > D.n) num_tile_items = (int)(D.n - tile_offset);
-
-  int keys[ITEMS_PER_THREAD];
-  int attr[ITEMS_PER_THREAD];
-  int vals[ITEMS_PER_THREAD];
-  int selection_flags[ITEMS_PER_THREAD];
-
-  BlockLoad<int, BLOCK_THREADS, ITEMS_PER_THREAD>(D.keys + tile_offset, keys, num_tile_items);
-  if (D.pred.n_steps > 0) BlockLoad<int, BLOCK_THREADS, ITEMS_PER_THREAD>(D.attr + tile_offset, attr, num_tile_items);
-  BlockLoad<int, BLOCK_THREADS, ITEMS_PER_THREAD>(D.payload + tile_offset, vals, num_tile_items);
-
-  unsigned long long n_sel = 0, n_oob = 0;
-#pragma unroll
-  for (int i = 0; i < ITEMS_PER_THREAD; i++) {
-    const bool in_tile = (int)(threadIdx.x + i * BLOCK_THREADS) < num_tile_items;
-    selection_flags[i] = in_tile && (D.pred.n_steps == 0 || eval_pred(D.pred, attr[i]));
-    n_sel += selection_flags[i];
-  }
-  if (D.bloom) BlockBuildBloomFilter<int, BLOCK_THREADS, ITEMS_PER_THREAD>(keys, selection_flags, D.bloom, D.bh,
-                                                                          num_tile_items);
-  // PHT_2 insert (BlockBuildSelectivePHT_2 with the fast remainder)
-#pragma unroll
-  for (int i = 0; i < ITEMS_PER_THREAD; i++) {
-    if (selection_flags[i]) {
-      const int64_t s = pht_slot(keys[i], D.ht_len, D.keys_min);
-      if (s < 0) {
-        n_oob++;
-        continue;
+  __shared__ unsigned int cnt[2];
+  if (a.clock && blockIdx.x == 0 && threadIdx.x == 0) *a.clock = (long long)wall_clock64();
+  const int64_t stride = (int64_t)gridDim.x * kBuildThreads;
+  const int64_t gtid = (int64_t)blockIdx.x * kBuildThreads + threadIdx.x;
+  for (int d = 0; d < a.n_dims; d++) {
+    const BuildDim& B = a.dim[d];
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t nsel = 0, noob = 0;
+    for (int64_t r = gtid; r < B.n; r += stride) {
+      const int32_t key = B.keys[r];
+      const bool sel = B.pred.n_steps == 0 || eval_pred(B.pred, B.attr[r]);
+      if (sel) {
+        nsel++;
+        if (B.bloom) bloom_insert(B.bh, B.bloom, key);
+        const int64_t s = pht_slot(key, B.ht_len, B.keys_min);
+        if (s < 0) {
+          noob++;
+        } else {
+          atomicCAS(&B.ht[2 * s], 0, key);
+          B.ht[2 * s + 1] = B.payload[r];
+        }
       }
-      atomicCAS(&D.ht[2 * s], 0, keys[i]);
-      D.ht[2 * s + 1] = vals[i];
     }
-  }
-  const unsigned long long sel_total = BlockSum<unsigned long long, BLOCK_THREADS, ITEMS_PER_THREAD>(n_sel, red);
-  const unsigned long long oob_total = BlockSum<unsigned long long, BLOCK_THREADS, ITEMS_PER_THREAD>(n_oob, red);
-  if (threadIdx.x == 0) {
-    if (sel_total) atomicAdd(&D.counters[0], sel_total);
-    if (oob_total) atomicAdd(&D.counters[1], oob_total);
-  }
-}
-
-hipError_t launch_build(const BuildArgs& a, int64_t n_blocks, hipStream_t s) {
-  if (n_blocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(build_kernel, dim3((unsigned)n_blocks), dim3(kBuildThreads), 0, s, a);
-  return hipGetLastError();
-}
-
-int64_t build_tile_rows() { return (int64_t)kBuildThreads * kBuildItems; }
-
-// ------------------------------------------------------------ finalize ----
-// One workgroup: copy the touched groups out in touch order, reset their
-// dense accumulators to zero for the next query, reduce the per-workgroup
-// statistics of the probe.
-__global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs a) {
-  __shared__ uint32_t n_sh;
-  __shared__ long long part[1024 / kStats][kStats];
-  if (threadIdx.x == 0) n_sh = *a.n_touched;
-  // statistics: thread (r, k) sums word k over blocks r, r + 64, ...; then a
-  // column sum over the 64 partial rows
-  {
-    const int k = threadIdx.x % kStats, r = threadIdx.x / kStats;
-    long long acc = 0;
-    if (k != ST_ORDER)
-      for (int b = r; b < a.n_blocks; b += 1024 / kStats) acc += a.stats[(int64_t)b * kStats + k];
-    part[r][k] = acc;
-  }
-  __syncthreads();
-  const uint32_t n = n_sh;
-  const uint32_t listed = n < a.touched_cap ? n : a.touched_cap;
-  for (uint32_t i = threadIdx.x; i < listed; i += blockDim.x) {
-    const uint32_t g = a.touched[i];
-    if ((int64_t)i < a.out_cap) {
-      GroupOut o;
-      o.g = g;
-      o.sum = (long long)a.gsum[g];
-      o.count = a.gcount[g];
-      o.prod = a.gprod ? a.gprod[g] : 0ull;
-      for (int d = 0; d < kMaxDims; d++) o.pay[d] = a.gpay[(int64_t)g * kMaxDims + d];
-      a.out[i] = o;
+    nsel = WaveSum(nsel);
+    noob = WaveSum(noob);
+    if (__lane_id() == 0) {
+      if (nsel) atomicAdd(&cnt[0], nsel);
+      if (noob) atomicAdd(&cnt[1], noob);
     }
-    a.gsum[g] = 0;
-    a.gcount[g] = 0;
-    if (a.gprod) a.gprod[g] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (cnt[0]) atomicAdd(&B.counters[0], (unsigned long long)cnt[0]);
+      if (cnt[1]) atomicAdd(&B.counters[1], (unsigned long long)cnt[1]);
+    }
+    __syncthreads();
   }
-  for (int k = threadIdx.x; k < kStats; k += blockDim.x) {
-    long long acc = 0;
-    if (k == ST_ORDER) acc = a.n_blocks > 0 ? a.stats[k] : 0;
-    else
-      for (int r = 0; r < 1024 / kStats; r++) acc += part[r][k];
-    a.out_hdr[1 + k] = acc;
-  }
-  if (threadIdx.x == 0) a.out_hdr[0] = n;
-  if (threadIdx.x < 2 * kMaxDims) {
-    const int d = threadIdx.x / 2;
-    a.out_hdr[1 + kStats + threadIdx.x] = a.dim_counters[d] ? (long long)a.dim_counters[d][threadIdx.x % 2] : 0;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) *a.n_touched = 0;
 }
 
-hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, s, a);
+hipError_t launch_build(const BuildArgs& a, int n_cu, hipStream_t s) {
+  int64_t rows = 0;
+  for (int d = 0; d < a.n_dims; d++) rows = a.dim[d].n > rows ? a.dim[d].n : rows;
+  if (rows <= 0) return hipSuccess;
+  int64_t blocks = (rows + kBuildThreads * 4 - 1) / (kBuildThreads * 4);
+  if (blocks > (int64_t)n_cu * 8) blocks = (int64_t)n_cu * 8;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(build_kernel, dim3((unsigned)blocks), dim3(kBuildThreads), 0, s, a);
   return hipGetLastError();
 }
 

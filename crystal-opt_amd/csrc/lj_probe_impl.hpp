@@ -478,6 +478,54 @@ __device__ __forceinline__ void load_tile(const int32_t* col, uint32_t base, uin
   }
 }
 
+// Result collection by one workgroup (the last to finish the probe): copy the
+// touched groups out in touch order, reset their dense accumulators to zero
+// for the next query, reduce the per-workgroup statistics.  `part` is LDS
+// scratch of WG long longs.
+template <int WG>
+__device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* part) {
+  constexpr int ROWS = WG / kStats;
+  const uint32_t n = __hip_atomic_load(a.n_touched, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  {
+    const int k = threadIdx.x % kStats, r = threadIdx.x / kStats;
+    long long acc = 0;
+    if (k != ST_ORDER)
+      for (int b = r; b < (int)gridDim.x; b += ROWS) acc += a.stats[(int64_t)b * kStats + k];
+    part[r * kStats + k] = acc;
+  }
+  __syncthreads();
+  const uint32_t listed = n < a.touched_cap ? n : a.touched_cap;
+  for (uint32_t i = threadIdx.x; i < listed; i += WG) {
+    const uint32_t g = a.touched[i];
+    if ((int64_t)i < a.out_cap) {
+      GroupOut o;
+      o.g = g;
+      o.sum = (long long)a.gsum[g];
+      o.count = a.gcount[g];
+      o.prod = a.gprod[g];
+      for (int d = 0; d < kMaxDims; d++) o.pay[d] = a.gpay[(int64_t)g * kMaxDims + d];
+      a.out[i] = o;
+    }
+    a.gsum[g] = 0;
+    a.gcount[g] = 0;
+    a.gprod[g] = 0;
+  }
+  for (int k = threadIdx.x; k < kStats; k += WG) {
+    long long acc = 0;
+    if (k == ST_ORDER) acc = a.stats[k];
+    else
+      for (int r = 0; r < ROWS; r++) acc += part[r * kStats + k];
+    a.out_hdr[1 + k] = acc;
+  }
+  if (threadIdx.x == 0) a.out_hdr[0] = n;
+  if (threadIdx.x < 2 * kMaxDims) {
+    const int d = threadIdx.x / 2;
+    a.out_hdr[1 + kStats + threadIdx.x] = a.dim_counters[d] ? (long long)a.dim_counters[d][threadIdx.x % 2] : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *a.n_touched = 0;
+}
+
 struct TileCtx {
   uint32_t n;        // fact rows
   uint32_t start;    // first row of the tile space (after the adaptive sample)
@@ -622,6 +670,8 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   long long* lds_stats = reinterpret_cast<long long*>(lds_raw + off);
   off += kStats * 8;
   unsigned long long* lds_samp = reinterpret_cast<unsigned long long*>(lds_raw + off);  // [kMaxDims + 1]
+
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.out_hdr[kClockSlot + 1] = (long long)wall_clock64();
 
   // ---- stage the LDS-resident filters, clear the group table
   for (int d = 0; d < a.n_dims; d++) {
@@ -845,6 +895,24 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
     }
   }
   for (int i = threadIdx.x; i < kStats; i += WG) a.stats[(int64_t)blockIdx.x * kStats + i] = lds_stats[i];
+
+  // ---- the last workgroup to finish collects the result ---------------------------
+  // Each workgroup publishes its writes with ONE agent-scope release (the
+  // ticket); only the last one acquires.  A full __threadfence here would also
+  // invalidate the XCD's L2 for every finishing workgroup and drop the filters
+  // and hash tables that the still-running workgroups on that XCD are reading.
+  __syncthreads();
+  uint32_t* ticket = reinterpret_cast<uint32_t*>(lds_samp);
+  if (threadIdx.x == 0) *ticket = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (*ticket == gridDim.x - 1) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    finalize_block<WG>(a, reinterpret_cast<long long*>(lds_raw));
+    if (threadIdx.x == 0) {
+      *a.done = 0;
+      a.out_hdr[kClockSlot + 2] = (long long)wall_clock64();
+    }
+  }
 }
 
 // Host-side table of instantiated kernels, filled by the instantiation units.
