@@ -37,6 +37,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per probe launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE of the probe kernel, each scaled by the same
+    counter's reading for a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "probe_q33_120M_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(n_fact, seed, threads):
     """Oracle (C restatement of the reference semantics) on the host cores:
     the LIP plan in join order S->C->D, whole 120M-row input, median of 5
@@ -71,6 +83,8 @@ def main():
     ap.add_argument("--plan-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plans", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the star join shards by fact "
@@ -95,6 +109,17 @@ def main():
         if key not in descs:
             descs[key] = W.q33_star_desc(cols, n, method, order, bloom_order)
         return ctx.star_join(descs[key])
+
+    if args.calib_copy_mb:
+        # a plain copy of known size: read B bytes, write B bytes (scales the HBM counters)
+        words = args.calib_copy_mb * (1 << 20) // 4
+        src = torch.ones(words, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: 3 x {args.calib_copy_mb} MiB")
+        del src, dst
 
     # ---- per-plan comparison (naive vs LIP in every join order) -----------------
     plans = {}
@@ -156,7 +181,7 @@ def main():
         bytes_alg = 128 * (sum(st["lines_in"][:4]) + st["sample_lines"])
     achieved = bytes_alg / probe_avg_s
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None,
+                "frac": achieved / HBM_PEAK, "traffic": measured_traffic(),
                 "kernel": "lj::probe_kernel (fused LIP scan)", "bytes_alg_per_launch": bytes_alg,
                 "probe_ms_avg": probe_avg_s * 1e3}
 

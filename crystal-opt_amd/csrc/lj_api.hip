@@ -131,8 +131,11 @@ static lj_status dev_alloc(lj_ctx* c, size_t bytes, void** out) {
   if (e != hipSuccess) {
     // give the cache back and retry once
     (void)hipGetLastError();
+    (void)hipStreamSynchronize(c->stream);  // parked blocks may still be cleared by queued fills
     for (auto& kv : c->free_blocks) (void)hipFree(kv.second);
+    for (auto& kv : c->zero_blocks) (void)hipFree(kv.second);
     c->free_blocks.clear();
+    c->zero_blocks.clear();
     c->cached = 0;
     e = hipMalloc(&p, sz);
     if (e != hipSuccess) {
@@ -731,11 +734,6 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
     a.load_order[i] = i < nd ? (lip ? bloom_order[i] : q->order[i]) : i;
   }
   a.lds_filter_words = lds_words;
-  if (adaptive) {
-    const int64_t total_waves = grid * nw;
-    const int64_t nr = (q->n_fact + kSegRows - 1) / kSegRows * kSegRows;
-    a.sample_rows = std::min<int64_t>(total_waves * kSegRows, nr);
-  }
   a.goffset = q->group_offset;
   a.n_groups = q->n_groups;
   a.gsum = c->gsum;
@@ -763,6 +761,7 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
       return s;
     }
   }
+  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
   HIP_TRY(c, launch_probe(cfg, a, (int)grid, lds_bytes, c->stream));
   HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
 
@@ -801,15 +800,12 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
                          sizeof(GroupOut) * (size_t)(n_touched - kHostGroups), hipMemcpyDeviceToHost));
   }
   std::sort(qo.groups.begin(), qo.groups.end(), [](const GroupOut& x, const GroupOut& y) { return x.g < y.g; });
-  // phases from the kernels' own wall-clock stamps: build start, probe start, probe end
-  {
-    const double to_ms = 1.0 / c->clock_khz;
-    const long long t0 = hdr[kClockSlot], t1 = hdr[kClockSlot + 1], t2 = hdr[kClockSlot + 2];
-    qo.t_fill = 0.f;
-    qo.t_build = built_here ? (float)((t1 - t0) * to_ms) : 0.f;
-    qo.t_probe = (float)((t2 - t1) * to_ms);
-  }
+  // phases from HIP events on the query's stream: [build launch] [probe launch]
+  qo.t_fill = 0.f;
+  HIP_TRY(c, hipEventElapsedTime(&qo.t_build, c->ev[0], c->ev[3]));
+  HIP_TRY(c, hipEventElapsedTime(&qo.t_probe, c->ev[3], c->ev[1]));
   HIP_TRY(c, hipEventElapsedTime(&qo.t_total, c->ev[0], c->ev[1]));
+  if (!built_here) qo.t_build = 0.f;
   int64_t build_oob = 0;
   for (int d = 0; d < nd; d++) {
     if (q->resident[d]) {

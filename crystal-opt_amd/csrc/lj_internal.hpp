@@ -72,7 +72,6 @@ struct ProbeArgs {
   int32_t load_order[kMaxDims];   // column load order (NAIVE: order, LIP: Bloom order)
   int32_t lds_filter_words;
   int32_t pad0;
-  int64_t sample_rows;     // rows [0, sample_rows) are handled by the sampling pass
   int64_t goffset;
   int64_t n_groups;
   unsigned long long* gsum;

@@ -598,26 +598,132 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
   return total;
 }
 
-// Full tiles t = t0, t0 + stride, ... < n_full.  Every iteration issues the
-// same loads (the next tile, clamped to tile 0 past the end), so the wait for
-// the current tile is a fixed vmcnt and the prefetch stays in flight while the
-// current tile is processed.  EAGER also streams the next column.
-template <int VEC, int K0, bool EAGER>
+// Adaptive LIP: a wave's first tile is its sample.  Every FK column of the
+// tile is loaded and every Bloom filter applied to every row; the per-filter
+// pass counts go into the workgroup's LDS totals.  The wave then orders its
+// filters by the totals seen so far (its own sample at least; no barrier),
+// most selective first, and streams the next column with the first one when
+// the chosen first filter passes more than a quarter of the sampled rows.  The
+// tile itself is then processed like any other (its columns are in L2).
+template <int VEC>
+__device__ __forceinline__ void sample_tile(const ProbeArgs& a, const uint32_t* lds_filter, WaveCtx& w,
+                                            WaveStats& st, unsigned long long* lds_samp, uint32_t base,
+                                            uint32_t lane4, int64_t n64, bool& eager) {
+  uint32_t cnt[kMaxDims] = {0, 0, 0, 0};
+  uint32_t n_valid = 0;
+#pragma unroll
+  for (int v = 0; v < VEC; v++) {
+    const uint32_t row0 = base + v * kSegRows + lane4;
+    uint32_t valid = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) valid |= (uint32_t)((int64_t)row0 + j < n64) << j;
+    // one column at a time (a rolled loop keeps the register peak of this
+    // once-per-wave step below the streaming loop's)
+#pragma unroll 1
+    for (int d = 0; d < w.nd; d++) {
+      const v4i32 kk = WaveLoadVec4<false>(a.dim[d].fk, row0, n64);
+      uint32_t m = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const bool vj = (valid >> j) & 1u;
+        m |= (uint32_t)(vj && bloom_probe_rt(a.dim[d], lds_filter, vget(kk, j))) << j;
+      }
+      const uint32_t c = popc_ballot4(m);
+      cnt[0] += d == 0 ? c : 0u;
+      cnt[1] += d == 1 ? c : 0u;
+      cnt[2] += d == 2 ? c : 0u;
+      cnt[3] += d == 3 ? c : 0u;
+    }
+    const int nv = popc_ballot4(valid);
+    n_valid += nv;
+    st.v[ST_SAMPLE_ROWS] += nv;
+    st.v[ST_SAMPLE_LINES] += (long long)w.nd * lines_of_ballot(__ballot(valid != 0));
+  }
+  // workgroup totals so far, including this wave's sample
+  uint32_t* samp = reinterpret_cast<uint32_t*>(lds_samp);
+  uint32_t tot[kMaxDims + 1] = {0, 0, 0, 0, 0};
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int d = 0; d < kMaxDims; d++)
+      if (d < w.nd) tot[d] = atomicAdd(&samp[d], cnt[d]) + cnt[d];
+    tot[kMaxDims] = atomicAdd(&samp[kMaxDims], n_valid) + n_valid;
+  }
+#pragma unroll
+  for (int d = 0; d <= kMaxDims; d++) tot[d] = (uint32_t)__builtin_amdgcn_readfirstlane((int)tot[d]);
+  if (tot[kMaxDims] == 0) return;
+  // rank of each dimension by pass count (ties: lower index first)
+  uint32_t cntd[kMaxDims];
+#pragma unroll
+  for (int d = 0; d < kMaxDims; d++) cntd[d] = d < w.nd ? tot[d] : 0xFFFFFFFFu;
+  int rank[kMaxDims];
+#pragma unroll
+  for (int d = 0; d < kMaxDims; d++) {
+    rank[d] = 0;
+#pragma unroll
+    for (int e = 0; e < kMaxDims; e++) rank[d] += (cntd[e] < cntd[d]) || (cntd[e] == cntd[d] && e < d);
+  }
+  int o[kMaxDims];
+#pragma unroll
+  for (int s2 = 0; s2 < kMaxDims; s2++) {
+    o[s2] = 0;
+#pragma unroll
+    for (int d = 0; d < kMaxDims; d++) o[s2] += rank[d] == s2 ? d : 0;
+  }
+  w.ld0 = uniform(o[0]);
+  w.ld1 = uniform(o[1]);
+  w.ld2 = uniform(o[2]);
+  w.ld3 = uniform(o[3]);
+  const uint32_t c0 = o[0] == 0 ? cntd[0] : o[0] == 1 ? cntd[1] : o[0] == 2 ? cntd[2] : cntd[3];
+  eager = (uint64_t)c0 * 4 > tot[kMaxDims];
+}
+
+// Tiles handed out inside a workgroup.  Workgroup b owns the full tiles
+// r * stride + b * NW + i (rounds r, i < NW), the same set a static
+// wave-per-tile assignment would give it; its waves take them in order from
+// an LDS counter, so a wave that runs ahead (SIMD arbitration favours some
+// waves) takes more tiles and the workgroup's waves finish together.  Slot j
+// of the workgroup maps to tile (j / NW) * stride + b * NW + j % NW; slots
+// 0..NW-1 are the waves' first tiles (taken without the counter).
+template <int NW>
+struct TileSched {
+  uint32_t* next;    // LDS: slots taken after the first round
+  uint32_t stride;   // waves in the grid
+  uint32_t wg_base;  // b * NW
+  // Issue the take; the value is read (readfirstlane) by tile() later, so the
+  // LDS atomic's latency hides behind the tile in flight.
+  __device__ __forceinline__ uint32_t issue() const {
+    uint32_t j = 0;
+    if (lane_id() == 0) j = atomicAdd(next, 1u);
+    return j;
+  }
+  __device__ __forceinline__ uint32_t tile(uint32_t raw) const {
+    const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)raw) + NW;
+    return (j / NW) * stride + wg_base + (j % NW);
+  }
+};
+
+// Full tiles from the workgroup's pool until it is empty.  Every iteration
+// issues the same loads (the next tile, clamped to tile 0 past the end), so
+// the wait for the current tile is a fixed vmcnt and the prefetch stays in
+// flight while the current tile is processed.  EAGER also streams the next
+// column.  Tiles come in increasing order per wave (the counter only grows).
+template <int VEC, int K0, bool EAGER, int NW>
 __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, const uint32_t* lds, GroupLds& gt,
-                                            WaveCtx& w, WaveStats& st, const TileCtx& tc, uint32_t t0,
-                                            uint32_t n_full, uint32_t stride, int& oob) {
+                                            WaveCtx& w, WaveStats& st, const TileCtx& tc, const TileSched<NW>& sch,
+                                            uint32_t n_full, int& oob) {
   constexpr uint32_t TILE = (uint32_t)VEC * kSegRows;
-  if (t0 >= n_full) return;
+  uint32_t t = sch.tile(sch.issue());
+  if (t >= n_full) return;
   v4i32 cur[VEC], cure[VEC];
 #pragma unroll
   for (int v = 0; v < VEC; v++) {
-    const uint32_t r = tc.start + t0 * TILE + v * kSegRows + tc.lane4;
+    const uint32_t r = tc.start + t * TILE + v * kSegRows + tc.lane4;
     if (EAGER) cure[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(tc.eager_col + r));
     else cure[v] = v4i32{0, 0, 0, 0};
     cur[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(s0.fk + r));
   }
-  for (uint32_t t = t0; t < n_full; t += stride) {
-    const uint32_t tn = t + stride;
+  uint32_t tn = sch.tile(sch.issue());
+  for (;;) {
     const uint32_t tl = tn < n_full ? tn : 0u;
     v4i32 nxt[VEC], nxte[VEC];
 #pragma unroll
@@ -627,10 +733,14 @@ __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, co
       else nxte[v] = v4i32{0, 0, 0, 0};
       nxt[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(s0.fk + r));
     }
+    const uint32_t raw = sch.issue();
     // keep the prefetch ahead of the first use of `cur` (the wait below is then
     // vmcnt(#prefetch loads) instead of vmcnt(0))
     __builtin_amdgcn_sched_barrier(0);
     tile_step<VEC, K0, false>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
+    if (tn >= n_full) break;
+    t = tn;
+    tn = sch.tile(raw);
 #pragma unroll
     for (int v = 0; v < VEC; v++) {
       cur[v] = nxt[v];
@@ -651,7 +761,7 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
 #endif
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   constexpr int NW = WG / 64;
-  const int wave = threadIdx.x >> 6;
+  const int wave = uniform(threadIdx.x >> 6);  // known wave-uniform: keeps per-wave branches scalar
   const int lane = (int)lane_id();
 
   uint32_t* lds_filter = reinterpret_cast<uint32_t*>(lds_raw);
@@ -670,6 +780,8 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   long long* lds_stats = reinterpret_cast<long long*>(lds_raw + off);
   off += kStats * 8;
   unsigned long long* lds_samp = reinterpret_cast<unsigned long long*>(lds_raw + off);  // [kMaxDims + 1]
+  off += (kMaxDims + 1) * 8;
+  uint32_t* lds_next = reinterpret_cast<uint32_t*>(lds_raw + off);  // tile counter
 
   if (blockIdx.x == 0 && threadIdx.x == 0) a.out_hdr[kClockSlot + 1] = (long long)wall_clock64();
 
@@ -696,6 +808,7 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   }
   for (int i = threadIdx.x; i < kStats; i += WG) lds_stats[i] = 0;
   for (int i = threadIdx.x; i <= kMaxDims; i += WG) lds_samp[i] = 0;
+  if (threadIdx.x == 0) *lds_next = 0;
   __syncthreads();
 
   WaveStats st;
@@ -720,105 +833,13 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   int oob = 0;
   bool eager = false;  // prefetch the next column with the first one
 
-  // ---- adaptive: sample one segment per wave with every filter --------------
-  if (a.adaptive) {
-    const uint32_t row0 = gw * kSegRows + lane * 4;
-    const bool has = (int64_t)gw * kSegRows < a.sample_rows;
-    v4i32 kk[kMaxDims];
-    uint32_t pass[kMaxDims];
-    uint32_t valid = 0;
-#pragma unroll
-    for (int d = 0; d < kMaxDims; d++) {
-      kk[d] = v4i32{0, 0, 0, 0};
-      pass[d] = 0;
-    }
-    if (has) {
-#pragma unroll
-      for (int j = 0; j < 4; j++) valid |= (uint32_t)((int64_t)row0 + j < n64) << j;
-#pragma unroll
-      for (int d = 0; d < kMaxDims; d++) {
-        if (d >= w.nd) continue;
-        kk[d] = WaveLoadVec4<false>(a.dim[d].fk, row0, n64);
-        uint32_t m = 0, bad = 0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int32_t key = vget(kk[d], j);
-          const bool vj = (valid >> j) & 1u;
-          bad |= (uint32_t)(vj && key < a.dim[d].keys_min) << j;
-          m |= (uint32_t)(vj && bloom_probe_rt(a.dim[d], lds_filter, key)) << j;
-        }
-        pass[d] = m;
-        st.v[ST_OOB] += popc_ballot4(bad);
-        const int c = popc_ballot4(m);
-        if (lane == 0) atomicAdd(&lds_samp[d], (unsigned long long)c);
-      }
-      const int nv = popc_ballot4(valid);
-      if (lane == 0) atomicAdd(&lds_samp[kMaxDims], (unsigned long long)nv);
-      st.v[ST_SAMPLE_ROWS] += nv;
-      st.v[ST_SAMPLE_LINES] += (long long)w.nd * lines_of_ballot(__ballot(valid != 0));
-    }
-    __syncthreads();
-    const unsigned long long n_sampled = lds_samp[kMaxDims];
-    if (n_sampled > 0) {
-      // rank of each dimension by sampled pass count (ties: lower index first)
-      unsigned long long cntd[kMaxDims];
-#pragma unroll
-      for (int d = 0; d < kMaxDims; d++) cntd[d] = d < w.nd ? lds_samp[d] : ~0ull;
-      int rank[kMaxDims];
-#pragma unroll
-      for (int d = 0; d < kMaxDims; d++) {
-        rank[d] = 0;
-#pragma unroll
-        for (int e = 0; e < kMaxDims; e++) rank[d] += (cntd[e] < cntd[d]) || (cntd[e] == cntd[d] && e < d);
-      }
-      int o[kMaxDims];
-#pragma unroll
-      for (int s2 = 0; s2 < kMaxDims; s2++) {
-        o[s2] = 0;
-#pragma unroll
-        for (int d = 0; d < kMaxDims; d++) o[s2] += rank[d] == s2 ? d : 0;
-      }
-      w.ld0 = uniform(o[0]);
-      w.ld1 = uniform(o[1]);
-      w.ld2 = uniform(o[2]);
-      w.ld3 = uniform(o[3]);
-      // start eager when the chosen first filter passes more than a quarter of the sample
-      const unsigned long long c0 = o[0] == 0 ? cntd[0] : o[0] == 1 ? cntd[1] : o[0] == 2 ? cntd[2] : cntd[3];
-      eager = c0 * 4 > n_sampled;
-    }
-    if (has) {
-      // finish the sampled rows in place: every Bloom filter is already applied
-      uint32_t mask = valid;
-#pragma unroll
-      for (int d = 0; d < kMaxDims; d++)
-        if (d < w.nd) mask &= pass[d];
-      st.v[ST_BLOOM_PASS] += popc_ballot4(mask);
-      Quad pay[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) pay[j].clear();
-#pragma unroll
-      for (int t = 0; t < kMaxDims; t++) {
-        if (t >= w.nd) continue;
-        const int e = a.order[t];
-        const v4i32 ke = e == 0 ? kk[0] : e == 1 ? kk[1] : e == 2 ? kk[2] : kk[3];
-        uint32_t nm = 0;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          if ((mask >> j) & 1u) {
-            int32_t p = 0;
-            const bool ok = pht_probe(a.dim[e], vget(ke, j), p);
-            pay[j].set(e, p);
-            nm |= (uint32_t)ok << j;
-          }
-        }
-        mask = nm;
-      }
-      st.v[ST_SAMPLE_LINES] += lines_of_ballot(__ballot(mask != 0));
-      const v4i32 mv = WavePredLoadVec4(a.measure, row0, n64, mask);
-#pragma unroll
-      for (int j = 0; j < 4; j++) st.v[ST_OOB] += group_add(a, gt, (mask >> j) & 1u, pay[j], vget(mv, j));
-    }
-  }
+  // ---- first tile: the adaptive sample --------------------------------------------
+  constexpr uint32_t TILE = (uint32_t)VEC * kSegRows;
+  const uint32_t n_rows = (uint32_t)n64;
+  const uint32_t n_full = n_rows / TILE;
+  const uint32_t n_tiles = (n_rows + TILE - 1) / TILE;
+  if (a.adaptive && gw < n_tiles)
+    sample_tile<VEC>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, eager);
 
   // ---- streaming loop over tiles -------------------------------------------------
   const ProbeDim& D0 = a.dim[w.ld0];
@@ -835,14 +856,10 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   const int32_t* eager_col = w.nd > 1 ? a.dim[w.ld1].fk : a.measure;
 
   TileCtx tc;
-  tc.n = (uint32_t)n64;
-  tc.start = (uint32_t)a.sample_rows;
+  tc.n = n_rows;
+  tc.start = 0;
   tc.lane4 = (uint32_t)lane * 4;
   tc.eager_col = eager_col;
-  constexpr uint32_t TILE = (uint32_t)VEC * kSegRows;
-  const uint32_t rest = tc.n > tc.start ? tc.n - tc.start : 0u;
-  const uint32_t n_full = rest / TILE;
-  const uint32_t n_tiles = (rest + TILE - 1) / TILE;
   // The wave's first tile and the table's last, partial tile go through the
   // checked path; the first tile's pass rate (or the adaptive sample) decides
   // whether the full-tile loop prefetches the next column too.
@@ -863,8 +880,12 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
       if (pass == 0 && !a.adaptive) eager = (uint32_t)tot * 4 > nv;
     }
     if (pass == 0) {
-      if (eager) stream_loop<VEC, K0, true>(a, s0, lds_filter, gt, w, st, tc, gw + total_waves, n_full, total_waves, oob);
-      else stream_loop<VEC, K0, false>(a, s0, lds_filter, gt, w, st, tc, gw + total_waves, n_full, total_waves, oob);
+      TileSched<NW> sch;
+      sch.next = lds_next;
+      sch.stride = total_waves;
+      sch.wg_base = blockIdx.x * NW;
+      if (eager) stream_loop<VEC, K0, true, NW>(a, s0, lds_filter, gt, w, st, tc, sch, n_full, oob);
+      else stream_loop<VEC, K0, false, NW>(a, s0, lds_filter, gt, w, st, tc, sch, n_full, oob);
     }
   }
   if (w.qcount > 0) process_batch(a, lds_filter, gt, w, w.qcount, st);

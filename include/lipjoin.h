@@ -154,10 +154,11 @@ typedef struct {
   int64_t rows_in[LJ_MAX_DIMS + 1];   /* rows alive when stage s loads its column */
   int64_t lines_in[LJ_MAX_DIMS + 1];  /* 32-row (128 B) lines holding >= 1 such row */
   int64_t rows_bloom_pass;            /* LIP: rows that passed every Bloom filter */
-  int64_t sample_rows;                /* rows handled by the adaptive sampling pass
-                                         (all columns read; not in rows_in/lines_in) */
-  int64_t sample_lines;               /* lines read by that pass, all columns */
-  int32_t order_used[LJ_MAX_DIMS];    /* the order stage 0..n-1 used (adaptive: of workgroup 0) */
+  int64_t sample_rows;                /* adaptive: rows of the sampled tiles (each wave's first
+                                         tile; all columns read; not in rows_in/lines_in) */
+  int64_t sample_lines;               /* lines read for the sampled tiles, all columns */
+  int32_t order_used[LJ_MAX_DIMS];    /* the order stage 0..n-1 used (adaptive: by the first
+                                         wave of workgroup 0; each wave orders its own) */
   int64_t n_oob;                      /* FK / group indices outside their domains */
 } lj_probe_stats;
 

@@ -174,7 +174,8 @@ def test_q33_plan_parity(ctx, q33_small, plan):
         if method == "lip":
             assert st["rows_bloom_pass"] == rows[4]
     else:
-        assert st["rows_in"][0] + st["sample_rows"] == n
+        assert st["rows_in"][0] == n
+        assert 0 < st["sample_rows"] <= n
         assert sorted(st["order_used"][:3]) == [0, 1, 2]
 
 
