@@ -11,10 +11,12 @@
 namespace lj {
 
 constexpr int kMaxDims = 4;
-constexpr int kGroupSlots = 64;      // LDS-privatised groups per workgroup
-constexpr int kQueue = 64;           // survivor queue entries per wave
+constexpr int kGroupSlots = 32;      // LDS-privatised groups per workgroup
+constexpr int kQueue = 64;           // survivor rows per batch (one per lane)
+constexpr int kQueueCap = kQueue + 8; // survivor queue entries per wave
 constexpr int kStats = 16;           // per-workgroup statistics words
 constexpr int kSegRows = 256;        // rows of one wave segment (64 lanes x 4)
+constexpr int kPrefetch = 1;         // full tiles a wave keeps in flight ahead of the one it processes
 // result header words: [0] groups hit, [1..kStats] statistics, then 2 counters
 // per dimension, then 3 kernel wall-clock stamps (start, build done, end)
 constexpr int kClockSlot = 1 + kStats + 2 * kMaxDims;

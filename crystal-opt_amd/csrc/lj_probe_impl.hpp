@@ -44,7 +44,11 @@
 
 namespace lj {
 
-constexpr int kDenseMin = 16;        // stage-0 survivors of a segment above which it is processed in place
+constexpr int kDenseMin = 8;         // stage-0 survivors of a segment above which it is processed in place
+constexpr int kGroupSlotBits = 5;
+static_assert((1 << kGroupSlotBits) == kGroupSlots, "group slots: a power of two");
+static_assert(kQueue + kDenseMin <= kQueueCap, "below kQueue entries, a sparse segment always fits");
+static_assert((kQueueCap - kQueue) + 16 * 4 <= kQueueCap, "after a rotation, a dense part always fits");
 constexpr int32_t kEmptyGroup = -1;
 
 // Wave-uniform counters (32-bit: one wave never sees 2^32 rows); indices into
@@ -245,7 +249,7 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
   if (!alive || g < 0) return n_bad;
   const unsigned long long p =
       a.want_product ? (unsigned long long)(long long)(int32_t)((uint32_t)m * (uint32_t)pay.v[0]) : 0ull;
-  uint32_t slot = ((uint32_t)g * 0x9E3779B1u) >> 26;  // 64 slots
+  uint32_t slot = ((uint32_t)g * 0x9E3779B1u) >> (32 - kGroupSlotBits);
   for (int probe = 0; probe < kGroupSlots; probe++) {
     int32_t k = gt.key[slot];
     if (k == kEmptyGroup) {
@@ -297,6 +301,11 @@ struct WaveCtx {
   int32_t* q_pay;
   int qcount;
   int32_t last1, last2, last3, last_meas;
+  // the batch taken from the queue whose first gather is in flight
+  int p_cnt;        // rows in it (0: none)
+  int32_t p_row;    // lane's entry: row | bit 31 = stage 1 done
+  int32_t p_x0, p_x1;
+  int32_t p_k;      // key gathered for the entry's first remaining stage
   __device__ __forceinline__ int ld(int s) const { return s == 0 ? ld0 : s == 1 ? ld1 : s == 2 ? ld2 : ld3; }
   __device__ __forceinline__ int32_t last(int s) const { return s == 1 ? last1 : s == 2 ? last2 : last3; }
   __device__ __forceinline__ void set_last(int s, int32_t x) {
@@ -306,22 +315,59 @@ struct WaveCtx {
   }
 };
 
-// Process `cnt` queued stage-0 survivors (lane i takes entry i).
-__device__ __forceinline__ void process_batch(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
-                                              int cnt, WaveStats& st) {
+// Queued rows are processed in batches of up to 64 (lane i takes entry i),
+// software-pipelined: issue_batch takes the entries out of the queue into
+// registers and starts the gather of each row's next FK column; the stream
+// goes on, and complete_batch, called when the queue fills again (or at the
+// end), finishes the rows, so the gather's latency hides behind tiles.
+//
+// An entry is (row, x0, x1): x is the key (LIP: the hash probes come last)
+// or the payload (naive) of the stage's dimension; x0 is stage 0's, and x1
+// stage 1's when bit 31 of row says the row passed stage 1 already (the
+// survivors of a dense segment).
+__device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int cnt) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int i = (int)lane_id();
-  bool alive = i < cnt;
-  int32_t row = 0;
+  const bool alive = i < cnt;
+  int32_t r = 0, x0 = 0, x1 = 0, k = 0;
+  if (alive) {
+    r = w.q_row[i];
+    x0 = w.q_key[i];
+    x1 = w.q_pay[i];
+  }
+  const int first = r < 0 ? 2 : 1;
+  if (alive && first < w.nd) {
+    const int32_t* col = first == 1 ? a.dim[w.ld1].fk : a.dim[w.ld2].fk;
+    k = col[(uint32_t)(r & 0x7FFFFFFF)];
+  }
+  w.p_cnt = cnt;
+  w.p_row = r;
+  w.p_x0 = x0;
+  w.p_x1 = x1;
+  w.p_k = k;
+}
+
+template <bool LIP>
+__device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
+                                               WaveStats& st) {
+  if (w.p_cnt == 0) return;
+  const int i = (int)lane_id();
+  bool alive = i < w.p_cnt;
+  const bool done1 = alive && w.p_row < 0;
+  const int32_t row = w.p_row & 0x7FFFFFFF;
+  const int first = done1 ? 2 : 1;
   Quad key, pay;
   key.clear();
   pay.clear();
   if (alive) {
-    row = w.q_row[i];
-    key.set(w.ld0, w.q_key[i]);
-    pay.set(w.ld0, w.q_pay[i]);
+    if (LIP) key.set(w.ld0, w.p_x0);
+    else pay.set(w.ld0, w.p_x0);
+    if (done1) {
+      if (LIP) key.set(w.ld1, w.p_x1);
+      else pay.set(w.ld1, w.p_x1);
+    }
   }
   const int nd = w.nd;
 #pragma unroll
@@ -329,16 +375,17 @@ __device__ __forceinline__ void process_batch(const ProbeArgs& a, const uint32_t
     if (s >= nd) continue;
     const int d = w.ld(s);
     const ProbeDim& D = a.dim[d];
-    st.v[ST_ROWS_IN + s] += __popcll(__ballot(alive));
+    const bool load = alive && s >= first;
+    st.v[ST_ROWS_IN + s] += __popcll(__ballot(load));
     int32_t ls = w.last(s);
-    st.v[ST_LINES_IN + s] += batch_lines(alive, row, ls);
+    st.v[ST_LINES_IN + s] += batch_lines(load, row, ls);
     w.set_last(s, ls);
     int32_t k = 0;
-    if (alive) k = D.fk[(uint32_t)row];
-    st.v[ST_OOB] += __popcll(__ballot(alive && k < D.keys_min));
-    if (alive) {
+    if (load) k = s == first ? w.p_k : D.fk[(uint32_t)row];
+    st.v[ST_OOB] += __popcll(__ballot(load && k < D.keys_min));
+    if (load) {
       key.set(d, k);
-      if (a.method == 0) {
+      if (!LIP) {
         int32_t p = 0;
         alive = pht_probe(D, k, p);
         pay.set(d, p);
@@ -347,7 +394,7 @@ __device__ __forceinline__ void process_batch(const ProbeArgs& a, const uint32_t
       }
     }
   }
-  if (a.method != 0) {
+  if (LIP) {
     st.v[ST_BLOOM_PASS] += __popcll(__ballot(alive));
 #pragma unroll
     for (int t = 0; t < kMaxDims; t++) {
@@ -365,15 +412,46 @@ __device__ __forceinline__ void process_batch(const ProbeArgs& a, const uint32_t
   int32_t m = 0;
   if (alive) m = a.measure[(uint32_t)row];
   st.v[ST_OOB] += group_add(a, gt, alive, pay, m);
+  w.p_cnt = 0;
+}
+
+// Finish the batch in flight and take the first kQueue queued rows as the
+// next one; the rest (fewer than kQueue) move to the front of the queue.
+template <bool LIP>
+__device__ __forceinline__ void rotate_batch(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
+                                             WaveStats& st) {
+  complete_batch<LIP>(a, lds, gt, w, st);
+  const int take = w.qcount < kQueue ? w.qcount : kQueue;
+  issue_batch(a, w, take);
+  const int rest = w.qcount - take;
+  if (rest > 0) {
+    // reads from [kQueue, kQueue + rest), writes to [0, rest): no overlap, and
+    // a wave's LDS operations complete in order (the batch read above first)
+    const int i = (int)lane_id();
+    if (i < rest) {
+      const int32_t r = w.q_row[kQueue + i], x0 = w.q_key[kQueue + i], x1 = w.q_pay[kQueue + i];
+      w.q_row[i] = r;
+      w.q_key[i] = x0;
+      w.q_pay[i] = x1;
+    }
+  }
+  w.qcount = rest;
 }
 
 // In-place processing of one segment (lane: rows row0..row0+3).  key0 / pay0:
 // stage-0 keys and payloads; mask: alive bits.  If use_eager, `eager` holds
 // the lane's 4 values of the next column already (stage 1's FK column, or the
-// measure column when there is a single dimension).
-__device__ __forceinline__ void process_dense(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
+// measure column when there is a single dimension).  With two or more
+// dimensions only stages 0 and 1 run here: the rest is left to the wave queue
+// (batches of 64 rows, so the later gathers of one segment do not each wait a
+// round trip): returns true with the stage-1 survivors in qmask and their
+// stage-0 / stage-1 keys (LIP) or payloads (naive) in qx0 / qx1.  With one
+// dimension the segment is finished here (returns false).
+template <bool LIP>
+__device__ __forceinline__ bool process_dense(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
                                               uint32_t row0, const v4i32& key0, const int32_t (&pay0)[4],
-                                              uint32_t mask, const v4i32& eager, bool use_eager, WaveStats& st) {
+                                              uint32_t mask, const v4i32& eager, bool use_eager, WaveStats& st,
+                                              uint32_t& qmask, int32_t (&qx0)[4], int32_t (&qx1)[4]) {
   const int nd = w.nd;
   const int64_t n = a.n;
   Quad key[4], pay[4];
@@ -404,7 +482,7 @@ __device__ __forceinline__ void process_dense(const ProbeArgs& a, const uint32_t
         const int32_t k = vget(kv, j);
         key[j].set(d, k);
         bool ok;
-        if (a.method == 0) {
+        if (!LIP) {
           int32_t p = 0;
           ok = pht_probe(D, k, p);
           pay[j].set(d, p);
@@ -415,8 +493,17 @@ __device__ __forceinline__ void process_dense(const ProbeArgs& a, const uint32_t
       }
     }
     mask = nm;
+    if (s == 1) {
+      qmask = mask;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        qx0[j] = LIP ? key[j].get(w.ld0) : pay[j].get(w.ld0);
+        qx1[j] = LIP ? key[j].get(d) : pay[j].get(d);
+      }
+      return true;
+    }
   }
-  if (a.method != 0) {
+  if (LIP) {
     st.v[ST_BLOOM_PASS] += popc_ballot4(mask);
 #pragma unroll
     for (int t = 0; t < kMaxDims; t++) {
@@ -442,20 +529,22 @@ __device__ __forceinline__ void process_dense(const ProbeArgs& a, const uint32_t
   else mv = WavePredLoadVec4(a.measure, row0, n, mask);
 #pragma unroll
   for (int j = 0; j < 4; j++) st.v[ST_OOB] += group_add(a, gt, (mask >> j) & 1u, pay[j], vget(mv, j));
+  return false;
 }
 
 // Append the alive rows of one segment to the wave queue (row order kept).
-__device__ __forceinline__ void enqueue(WaveCtx& w, uint32_t row0, const v4i32& k, const int32_t (&pay)[4],
-                                        uint32_t mask, int cnt_total) {
+// done1: the rows passed stage 1 too (x1 holds its key / payload).
+__device__ __forceinline__ void enqueue(WaveCtx& w, uint32_t row0, const int32_t (&x0)[4], const int32_t (&x1)[4],
+                                        uint32_t mask, bool done1, int cnt_total) {
   const unsigned long long b0 = __ballot(mask & 1u), b1 = __ballot(mask & 2u), b2 = __ballot(mask & 4u),
                            b3 = __ballot(mask & 8u);
   uint32_t pos = (uint32_t)w.qcount + mbcnt(b0) + mbcnt(b1) + mbcnt(b2) + mbcnt(b3);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     if ((mask >> j) & 1u) {
-      w.q_row[pos] = (int32_t)(row0 + j);
-      w.q_key[pos] = vget(k, j);
-      w.q_pay[pos] = pay[j];
+      w.q_row[pos] = (int32_t)((row0 + j) | (done1 ? 0x80000000u : 0u));
+      w.q_key[pos] = x0[j];
+      w.q_pay[pos] = x1[j];
       pos++;
     }
   }
@@ -541,6 +630,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
                                          WaveCtx& w, WaveStats& st, const TileCtx& tc, uint32_t base,
                                          const v4i32 (&k)[VEC], const v4i32 (&ke)[VEC], bool have_eager, int& oob) {
   constexpr uint32_t TILE = (uint32_t)VEC * kSegRows;
+  constexpr bool LIP = K0 != S0_PHT;  // a hash-table first stage is the naive plan
   uint32_t mask[VEC];
   int32_t pay[VEC][4];
 #pragma unroll
@@ -580,20 +670,35 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
     const int cnt = popc_ballot4(m);
     total += cnt;
     if (cnt == 0) continue;
+    // callers keep qcount < kQueue between tiles; later segments of a tile re-check
+    if (v > 0 && w.qcount >= kQueue) rotate_batch<LIP>(a, lds, gt, w, st);
     const uint32_t row0 = base + v * kSegRows + tc.lane4;
+    int32_t qx0[4], qx1[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      qx0[j] = LIP ? vget(kv, j) : pv[j];
+      qx1[j] = 0;
+    }
     if (cnt > kDenseMin || have_eager) {
-      process_dense(a, lds, gt, w, row0, kv, pv, m, ev, have_eager, st);
+      uint32_t qm = 0;
+      if (!process_dense<LIP>(a, lds, gt, w, row0, kv, pv, m, ev, have_eager, st, qm, qx0, qx1)) continue;
+      const int cq = popc_ballot4(qm);
+      if (cq == 0) continue;
+      // the stage-1 survivors (up to 4 per lane) go in up to four parts of
+      // 16 lanes each (at most 64 rows), in row order
+      const int parts = cq > kDenseMin ? 4 : 1;
+#pragma unroll 1
+      for (int part = 0; part < parts; part++) {
+        const uint32_t pm = parts == 1 ? qm : (((int)tc.lane4 >> 6) == part ? qm : 0u);
+        const int c = parts == 1 ? cq : popc_ballot4(pm);
+        if (c == 0) continue;
+        if (w.qcount + c > kQueueCap) rotate_batch<LIP>(a, lds, gt, w, st);
+        enqueue(w, row0, qx0, qx1, pm, true, c);
+      }
       continue;
     }
-    if (w.qcount + cnt > kQueue) {
-      process_batch(a, lds, gt, w, w.qcount, st);
-      w.qcount = 0;
-    }
-    enqueue(w, row0, kv, pv, m, cnt);
-    if (w.qcount == kQueue) {
-      process_batch(a, lds, gt, w, kQueue, st);
-      w.qcount = 0;
-    }
+    // a sparse segment adds at most kDenseMin rows: it fits (qcount < kQueue here)
+    enqueue(w, row0, qx0, qx1, m, false, cnt);
   }
   return total;
 }
@@ -702,11 +807,25 @@ struct TileSched {
   }
 };
 
-// Full tiles from the workgroup's pool until it is empty.  Every iteration
-// issues the same loads (the next tile, clamped to tile 0 past the end), so
-// the wait for the current tile is a fixed vmcnt and the prefetch stays in
-// flight while the current tile is processed.  EAGER also streams the next
-// column.  Tiles come in increasing order per wave (the counter only grows).
+// Full tiles from the workgroup's pool until it is empty, with kPrefetch
+// tiles in flight ahead of the one being processed.  Every iteration issues
+// the same loads (a tile past the end is clamped to tile 0), so the wait for
+// the current tile is a fixed vmcnt and the prefetches stay in flight while
+// it is processed.  EAGER also streams the next column.  Tiles come in
+// increasing order per wave (the counter only grows).
+template <int VEC, bool EAGER>
+__device__ __forceinline__ void load_full_tile(const S0& s0, const TileCtx& tc, uint32_t t, v4i32 (&k)[VEC],
+                                               v4i32 (&e)[VEC]) {
+  constexpr uint32_t TILE = (uint32_t)VEC * kSegRows;
+#pragma unroll
+  for (int v = 0; v < VEC; v++) {
+    const uint32_t r = tc.start + t * TILE + v * kSegRows + tc.lane4;
+    if (EAGER) e[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(tc.eager_col + r));
+    else e[v] = v4i32{0, 0, 0, 0};
+    k[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(s0.fk + r));
+  }
+}
+
 template <int VEC, int K0, bool EAGER, int NW>
 __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, const uint32_t* lds, GroupLds& gt,
                                             WaveCtx& w, WaveStats& st, const TileCtx& tc, const TileSched<NW>& sch,
@@ -715,36 +834,49 @@ __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, co
   uint32_t t = sch.tile(sch.issue());
   if (t >= n_full) return;
   v4i32 cur[VEC], cure[VEC];
-#pragma unroll
-  for (int v = 0; v < VEC; v++) {
-    const uint32_t r = tc.start + t * TILE + v * kSegRows + tc.lane4;
-    if (EAGER) cure[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(tc.eager_col + r));
-    else cure[v] = v4i32{0, 0, 0, 0};
-    cur[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(s0.fk + r));
-  }
+  load_full_tile<VEC, EAGER>(s0, tc, t, cur, cure);
   uint32_t tn = sch.tile(sch.issue());
-  for (;;) {
-    const uint32_t tl = tn < n_full ? tn : 0u;
-    v4i32 nxt[VEC], nxte[VEC];
+  if (kPrefetch == 1) {
+    for (;;) {
+      v4i32 nxt[VEC], nxte[VEC];
+      load_full_tile<VEC, EAGER>(s0, tc, tn < n_full ? tn : 0u, nxt, nxte);
+      const uint32_t raw = sch.issue();
+      // keep the prefetch ahead of the first use of `cur` (the wait below is
+      // then vmcnt(#prefetch loads) instead of vmcnt(0))
+      __builtin_amdgcn_sched_barrier(0);
+      tile_step<VEC, K0, false>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
+      if (w.qcount >= kQueue) rotate_batch<K0 != S0_PHT>(a, lds, gt, w, st);
+      if (tn >= n_full) break;
+      t = tn;
+      tn = sch.tile(raw);
 #pragma unroll
-    for (int v = 0; v < VEC; v++) {
-      const uint32_t r = tc.start + tl * TILE + v * kSegRows + tc.lane4;
-      if (EAGER) nxte[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(tc.eager_col + r));
-      else nxte[v] = v4i32{0, 0, 0, 0};
-      nxt[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(s0.fk + r));
+      for (int v = 0; v < VEC; v++) {
+        cur[v] = nxt[v];
+        cure[v] = nxte[v];
+      }
     }
-    const uint32_t raw = sch.issue();
-    // keep the prefetch ahead of the first use of `cur` (the wait below is then
-    // vmcnt(#prefetch loads) instead of vmcnt(0))
-    __builtin_amdgcn_sched_barrier(0);
-    tile_step<VEC, K0, false>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
-    if (tn >= n_full) break;
-    t = tn;
-    tn = sch.tile(raw);
+  } else {
+    v4i32 nx1[VEC], nx1e[VEC];
+    load_full_tile<VEC, EAGER>(s0, tc, tn < n_full ? tn : 0u, nx1, nx1e);
+    uint32_t raw2 = sch.issue();
+    for (;;) {
+      const uint32_t tnn = sch.tile(raw2);
+      v4i32 nx2[VEC], nx2e[VEC];
+      load_full_tile<VEC, EAGER>(s0, tc, tnn < n_full ? tnn : 0u, nx2, nx2e);
+      raw2 = sch.issue();
+      __builtin_amdgcn_sched_barrier(0);
+      tile_step<VEC, K0, false>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
+      if (w.qcount >= kQueue) rotate_batch<K0 != S0_PHT>(a, lds, gt, w, st);
+      if (tn >= n_full) break;
+      t = tn;
+      tn = tnn;
 #pragma unroll
-    for (int v = 0; v < VEC; v++) {
-      cur[v] = nxt[v];
-      cure[v] = nxte[v];
+      for (int v = 0; v < VEC; v++) {
+        cur[v] = nx1[v];
+        cure[v] = nx1e[v];
+        nx1[v] = nx2[v];
+        nx1e[v] = nx2e[v];
+      }
     }
   }
 }
@@ -766,8 +898,8 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
 
   uint32_t* lds_filter = reinterpret_cast<uint32_t*>(lds_raw);
   size_t off = ((size_t)a.lds_filter_words * 4 + 15) & ~(size_t)15;
-  int32_t* q_base = reinterpret_cast<int32_t*>(lds_raw + off) + wave * (3 * kQueue);
-  off += (size_t)NW * 3 * kQueue * 4;
+  int32_t* q_base = reinterpret_cast<int32_t*>(lds_raw + off) + wave * (3 * kQueueCap);
+  off += (size_t)NW * 3 * kQueueCap * 4;
   GroupLds gt;
   gt.cnt = reinterpret_cast<unsigned long long*>(lds_raw + off);
   gt.sum = gt.cnt + kGroupSlots;
@@ -818,10 +950,12 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   WaveCtx w;
   w.nd = a.n_dims;
   w.q_row = q_base;
-  w.q_key = q_base + kQueue;
-  w.q_pay = q_base + 2 * kQueue;
+  w.q_key = q_base + kQueueCap;
+  w.q_pay = q_base + 2 * kQueueCap;
   w.qcount = 0;
   w.last1 = w.last2 = w.last3 = w.last_meas = -1;
+  w.p_cnt = 0;
+  w.p_row = w.p_x0 = w.p_x1 = w.p_k = 0;
   w.ld0 = a.load_order[0];
   w.ld1 = a.load_order[1];
   w.ld2 = a.load_order[2];
@@ -838,8 +972,10 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   const uint32_t n_rows = (uint32_t)n64;
   const uint32_t n_full = n_rows / TILE;
   const uint32_t n_tiles = (n_rows + TILE - 1) / TILE;
-  if (a.adaptive && gw < n_tiles)
-    sample_tile<VEC>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, eager);
+  if constexpr (K0 != S0_PHT) {
+    if (a.adaptive && gw < n_tiles)
+      sample_tile<VEC>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, eager);
+  }
 
   // ---- streaming loop over tiles -------------------------------------------------
   const ProbeDim& D0 = a.dim[w.ld0];
@@ -876,6 +1012,7 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
         ke[v] = k[v];
       }
       const int tot = tile_step<VEC, K0, true>(a, s0, lds_filter, gt, w, st, tc, base, k, ke, false, oob);
+      if (w.qcount >= kQueue) rotate_batch<K0 != S0_PHT>(a, lds_filter, gt, w, st);
       const uint32_t nv = tc.n - base < TILE ? tc.n - base : TILE;
       if (pass == 0 && !a.adaptive) eager = (uint32_t)tot * 4 > nv;
     }
@@ -888,7 +1025,9 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
       else stream_loop<VEC, K0, false, NW>(a, s0, lds_filter, gt, w, st, tc, sch, n_full, oob);
     }
   }
-  if (w.qcount > 0) process_batch(a, lds_filter, gt, w, w.qcount, st);
+  // the batch in flight, then the rows still queued
+#pragma unroll 1
+  for (int it = 0; it < 2; it++) rotate_batch<K0 != S0_PHT>(a, lds_filter, gt, w, st);
 
   // ---- flush: statistics and groups ----------------------------------------------
   {
@@ -937,6 +1076,6 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
 }
 
 // Host-side table of instantiated kernels, filled by the instantiation units.
-#define LJ_PROBE_SHAPES(X, K) X(256, 1, K) X(256, 2, K) X(512, 1, K) X(512, 2, K) X(1024, 1, K) X(1024, 2, K)
+#define LJ_PROBE_SHAPES(X, K) X(256, 1, K) X(512, 1, K) X(1024, 1, K)
 
 }  // namespace lj
