@@ -21,7 +21,8 @@ def _short(name, width=72):
     return name if len(name) <= width else name[: width - 3] + "..."
 
 
-def trace(path, skip=0, match=None, out=sys.stdout):
+def trace(path, skip=0, match=None, out=None):
+    out = out or sys.stdout
     per = OrderedDict()
     with open(path) as f:
         for r in csv.DictReader(f):
@@ -42,7 +43,8 @@ def trace(path, skip=0, match=None, out=sys.stdout):
     return res
 
 
-def counters(path, skip=0, match=None, calib_bytes=0, out=sys.stdout):
+def counters(path, skip=0, match=None, calib_bytes=0, out=None):
+    out = out or sys.stdout
     per = OrderedDict()
     with open(path) as f:
         for r in csv.DictReader(f):
