@@ -393,14 +393,29 @@ static void dim_hashes(lj_dim* D) {
 
 // Return a dimension's tables to the zero pool; the clears are queued on the
 // stream (after whatever already used the tables).
-static void dim_free(lj_ctx* c, lj_dim* D) {
-  if (!D) return;
+// Release dimensions to the zero pool: one clearing launch for all of them.
+static void dims_free(lj_ctx* c, lj_dim* const* dims, int n) {
   FillArgs f{};
-  dev_release_zero(c, D->ht, f);
-  dev_release_zero(c, D->bloom, f);
-  dev_release_zero(c, D->counters, f);
+  for (int i = 0; i < n; i++) {
+    lj_dim* D = dims[i];
+    if (!D) continue;
+    dev_release_zero(c, D->ht, f);
+    dev_release_zero(c, D->bloom, f);
+    dev_release_zero(c, D->counters, f);
+    delete D;
+  }
   if (f.n) (void)launch_fill(f, c->stream);
-  delete D;
+}
+
+static void dim_free(lj_ctx* c, lj_dim* D) { dims_free(c, &D, 1); }
+
+// The counters of a dimension summed over their lanes: {selected, outside the domain}.
+static void sum_counters(const unsigned long long* h, unsigned long long out[2]) {
+  out[0] = out[1] = 0;
+  for (int l = 0; l < kCounterLanes; l++) {
+    out[0] += h[2 * l];
+    out[1] += h[2 * l + 1];
+  }
 }
 
 // Allocate the tables of a dimension (no launches).
@@ -413,7 +428,7 @@ static lj_status dim_alloc(lj_ctx* c, const lj_dim_desc* desc, lj_dim** out) {
   dim_hashes(D);
   if ((s = dev_alloc_zero_t(c, D->ht_words, &D->ht)) != LJ_OK ||
       (D->bloom_words && (s = dev_alloc_zero_t(c, D->bloom_words, &D->bloom)) != LJ_OK) ||
-      (s = dev_alloc_zero_t(c, 4, &D->counters)) != LJ_OK) {
+      (s = dev_alloc_zero_t(c, kCounterWords, &D->counters)) != LJ_OK) {
     dim_free(c, D);
     return s;
   }
@@ -454,7 +469,7 @@ static lj_status dims_launch(lj_ctx* c, lj_dim* const* dims, int n, long long* c
   BuildArgs b{};
   for (int i = 0; i < n; i++) set_build_dim(dims[i], b.dim[b.n_dims++]);
   b.clock = clock;
-  HIP_TRY(c, launch_build(b, c->n_cu, c->stream));
+  HIP_TRY(c, launch_build(b, c->stream));
   return LJ_OK;
 }
 
@@ -471,9 +486,10 @@ extern "C" lj_status lj_dim_build(lj_ctx* c, const lj_dim_desc* desc, lj_dim** o
     return s;
   }
   HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
-  unsigned long long h[2] = {0, 0};
-  HIP_TRY(c, hipMemcpyAsync(h, D->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long hc[2 * kCounterLanes], h[2];
+  HIP_TRY(c, hipMemcpyAsync(hc, D->counters, sizeof hc, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  sum_counters(hc, h);
   HIP_TRY(c, hipEventElapsedTime(&D->time_build_ms, c->ev[0], c->ev[1]));
   D->n_selected = (int64_t)h[0];
   D->n_oob = (int64_t)h[1];
@@ -493,7 +509,7 @@ extern "C" lj_status lj_dim_get_info(lj_ctx* c, const lj_dim* D, lj_dim_info* in
   info->bloom_words = D->bloom_words;
   info->time_build_ms = D->time_build_ms;
   if (D->bloom_words) {
-    unsigned long long* cnt = D->counters + 2;
+    unsigned long long* cnt = D->counters + 2 * kCounterLanes;
     HIP_TRY(c, hipMemsetAsync(cnt, 0, 8, c->stream));
     HIP_TRY(c, launch_popcount(D->bloom, D->bloom_words, cnt, c->stream));
     unsigned long long h = 0;
@@ -625,9 +641,7 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
       if (s == LJ_OK) owned[n_owned++] = dims[d];
     }
   }
-  auto cleanup = [&]() {
-    for (int i = 0; i < n_owned; i++) dim_free(c, owned[i]);
-  };
+  auto cleanup = [&]() { dims_free(c, owned, n_owned); };
   if (s != LJ_OK) {
     cleanup();
     return s;

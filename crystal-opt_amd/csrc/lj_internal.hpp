@@ -46,8 +46,15 @@ struct BuildDim {
   int32_t pad0;
   uint32_t* bloom;         // nullptr: no filter
   BloomHash bh;
-  unsigned long long* counters;  // [0] selected rows, [1] keys outside the domain
+  unsigned long long* counters;  // kCounterLanes x {selected rows, keys outside the domain}
+  int64_t block_begin;     // first workgroup of this dimension in the build launch
 };
+
+// A dimension's counters are spread over kCounterLanes pairs (workgroup w adds
+// to pair w % kCounterLanes) so that the build's per-workgroup atomics do not
+// all serialize on one address; readers sum the pairs.
+constexpr int kCounterLanes = 16;
+constexpr int kCounterWords = 2 * kCounterLanes + 2;  // + popcount scratch
 
 struct ProbeDim {
   const int32_t* fk;      // fact FK column of this dimension
@@ -128,7 +135,7 @@ struct GroupOut {
 
 // Host launchers (lj_kernels_*.hip).
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
-hipError_t launch_build(const BuildArgs& a, int n_cu, hipStream_t s);
+hipError_t launch_build(BuildArgs& a, hipStream_t s);
 hipError_t launch_gen(int32_t kind, uint64_t key, int64_t lo, int64_t hi, int64_t first, int64_t n,
                       int32_t* out, hipStream_t s);
 hipError_t launch_popcount(const uint32_t* words, int64_t n, unsigned long long* out, hipStream_t s);

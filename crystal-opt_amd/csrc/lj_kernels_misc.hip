@@ -52,21 +52,33 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
 // Predicate, Bloom insert, PHT_2 insert (CAS of the key, then the
 // unconditional value store) -- build_kernel of src/ops/bloom_join.cu:43-65 and
 // build_hashtable_{s,c,d} of src/ssb-jo/q33/0-lip.cu:114-203.  Every dimension
-// of the query in one launch: grid-stride over each dimension's rows in turn;
-// the selected / out-of-domain counts reduce per workgroup to one atomic each.
+// of the query in one launch: each dimension owns a range of workgroups (all
+// dimensions build at once), a thread takes 4 consecutive rows, and the
+// selected / out-of-domain counts reduce per workgroup to one atomic each.
 constexpr int kBuildThreads = 256;
+constexpr int kBuildRows = 4;
 
-__global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
+__global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs args) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)args;
+  const BuildArgs& a = *(const BuildArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
+#else
+  const BuildArgs& a = args;
+#endif
   __shared__ unsigned int cnt[2];
   if (a.clock && blockIdx.x == 0 && threadIdx.x == 0) *a.clock = (long long)wall_clock64();
-  const int64_t stride = (int64_t)gridDim.x * kBuildThreads;
-  const int64_t gtid = (int64_t)blockIdx.x * kBuildThreads + threadIdx.x;
-  for (int d = 0; d < a.n_dims; d++) {
-    const BuildDim& B = a.dim[d];
-    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t nsel = 0, noob = 0;
-    for (int64_t r = gtid; r < B.n; r += stride) {
+  int d = 0;
+  for (int k = 1; k < a.n_dims; k++)
+    if ((int64_t)blockIdx.x >= a.dim[k].block_begin) d = k;
+  const BuildDim& B = a.dim[d];
+  if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t r0 = (((int64_t)blockIdx.x - B.block_begin) * kBuildThreads + threadIdx.x) * kBuildRows;
+  uint32_t nsel = 0, noob = 0;
+#pragma unroll
+  for (int j = 0; j < kBuildRows; j++) {
+    const int64_t r = r0 + j;
+    if (r < B.n) {
       const int32_t key = B.keys[r];
       const bool sel = B.pred.n_steps == 0 || eval_pred(B.pred, B.attr[r]);
       if (sel) {
@@ -81,28 +93,30 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
         }
       }
     }
-    nsel = WaveSum(nsel);
-    noob = WaveSum(noob);
-    if (__lane_id() == 0) {
-      if (nsel) atomicAdd(&cnt[0], nsel);
-      if (noob) atomicAdd(&cnt[1], noob);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      if (cnt[0]) atomicAdd(&B.counters[0], (unsigned long long)cnt[0]);
-      if (cnt[1]) atomicAdd(&B.counters[1], (unsigned long long)cnt[1]);
-    }
-    __syncthreads();
+  }
+  nsel = WaveSum(nsel);
+  noob = WaveSum(noob);
+  if (__lane_id() == 0) {
+    if (nsel) atomicAdd(&cnt[0], nsel);
+    if (noob) atomicAdd(&cnt[1], noob);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long* c = B.counters + 2 * (blockIdx.x % kCounterLanes);
+    if (cnt[0]) atomicAdd(&c[0], (unsigned long long)cnt[0]);
+    if (cnt[1]) atomicAdd(&c[1], (unsigned long long)cnt[1]);
   }
 }
 
-hipError_t launch_build(const BuildArgs& a, int n_cu, hipStream_t s) {
-  int64_t rows = 0;
-  for (int d = 0; d < a.n_dims; d++) rows = a.dim[d].n > rows ? a.dim[d].n : rows;
-  if (rows <= 0) return hipSuccess;
-  int64_t blocks = (rows + kBuildThreads * 4 - 1) / (kBuildThreads * 4);
-  if (blocks > (int64_t)n_cu * 8) blocks = (int64_t)n_cu * 8;
-  if (blocks < 1) blocks = 1;
+// Assigns the workgroup ranges (block_begin) and launches.
+hipError_t launch_build(BuildArgs& a, hipStream_t s) {
+  constexpr int64_t rows_per_block = (int64_t)kBuildThreads * kBuildRows;
+  int64_t blocks = 0;
+  for (int d = 0; d < a.n_dims; d++) {
+    a.dim[d].block_begin = blocks;
+    blocks += (a.dim[d].n + rows_per_block - 1) / rows_per_block;
+  }
+  if (blocks <= 0) return hipSuccess;
   hipLaunchKernelGGL(build_kernel, dim3((unsigned)blocks), dim3(kBuildThreads), 0, s, a);
   return hipGetLastError();
 }

@@ -609,7 +609,10 @@ __device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* pa
   if (threadIdx.x == 0) a.out_hdr[0] = n;
   if (threadIdx.x < 2 * kMaxDims) {
     const int d = threadIdx.x / 2;
-    a.out_hdr[1 + kStats + threadIdx.x] = a.dim_counters[d] ? (long long)a.dim_counters[d][threadIdx.x % 2] : 0;
+    long long v = 0;
+    if (a.dim_counters[d])
+      for (int l = 0; l < kCounterLanes; l++) v += (long long)a.dim_counters[d][2 * l + threadIdx.x % 2];
+    a.out_hdr[1 + kStats + threadIdx.x] = v;
   }
   __syncthreads();
   if (threadIdx.x == 0) *a.n_touched = 0;
