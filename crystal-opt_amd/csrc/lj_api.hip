@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -78,6 +79,11 @@ struct lj_ctx {
   int probe_vec = 0;
   int lds_policy = -1;
   double clock_khz = 100000.0;  // device wall clock (kernel phase timestamps)
+  // Bloom autotuner cost model, per fact row, relative to the scan of an LDS filter
+  double tune_scan_gl = 1.5;     // scan with the first filter in L2 instead of LDS
+  double tune_probe_extra = 0.2; // each further bit probe of a standard k > 1 filter
+  double tune_surv = 40.0;       // a row that passes the filter (queue, gathers, hash probe)
+  int tune_calibrated = 0;
   std::map<std::pair<int, size_t>, int> occupancy;  // (config key, LDS bytes) -> workgroups per CU
 };
 
@@ -231,6 +237,8 @@ extern "C" int64_t lj_abi_struct_size(int32_t which) {
     case 5: return sizeof(lj_star_desc);
     case 6: return sizeof(lj_group);
     case 7: return sizeof(lj_star_result);
+    case 8: return sizeof(lj_batch_opts);
+    case 9: return sizeof(lj_bloom_choice);
   }
   return -1;
 }
@@ -606,19 +614,45 @@ struct QueryOut {
   int64_t dim_oob[kMaxDims];
 };
 
-// The common path of lj_star_join and lj_hash_join.
-static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, QueryOut& qo) {
+static lj_status occupancy_of(lj_ctx* c, const ProbeConfig& cfg, size_t lds_bytes, int* per_cu) {
+  const auto key = std::make_pair(cfg.wg * 1000 + cfg.vec * 100 + cfg.k0, lds_bytes);
+  auto it = c->occupancy.find(key);
+  if (it == c->occupancy.end()) {
+    HIP_TRY(c, probe_occupancy(cfg, lds_bytes, per_cu));
+    c->occupancy[key] = *per_cu;
+  } else {
+    *per_cu = it->second;
+  }
+  if (*per_cu < 1)
+    return fail(c, LJ_EUNSUPPORTED, "probe configuration wg=%d vec=%d with %zu B LDS does not fit", cfg.wg, cfg.vec,
+                lds_bytes);
+  return LJ_OK;
+}
+
+// ---- query planning (shared by single queries and batches) ----------------------
+struct Plan {
+  int nd = 0;
+  int32_t bloom_order[LJ_MAX_DIMS] = {0, 1, 2, 3};
+  lj_dim* dims[LJ_MAX_DIMS] = {nullptr, nullptr, nullptr, nullptr};
+  lj_dim* owned[LJ_MAX_DIMS] = {nullptr, nullptr, nullptr, nullptr};
+  int n_owned = 0;
+  ProbeConfig cfg{};
+  ProbeArgs a{};     // everything but the per-query buffers
+  size_t lds_bytes = 0;
+  int per_cu = 0;
+};
+
+static lj_status validate_query(lj_ctx* c, const lj_star_desc* q, Plan& p) {
   if (!q) return fail(c, LJ_EINVAL, "query descriptor is NULL");
   const int nd = q->n_dims;
   if (nd < 1 || nd > LJ_MAX_DIMS) return fail(c, LJ_EINVAL, "n_dims %d not in 1..%d", nd, LJ_MAX_DIMS);
   if (q->n_fact < 0 || q->n_fact >= (1ll << 31) - 4096) return fail(c, LJ_EINVAL, "n_fact %lld out of range", (long long)q->n_fact);
   if (q->method < LJ_METHOD_NAIVE || q->method > LJ_METHOD_LIP_ADAPTIVE) return fail(c, LJ_EINVAL, "unknown method %d", q->method);
   if (!is_perm(q->order, nd)) return fail(c, LJ_EINVAL, "order is not a permutation of 0..%d", nd - 1);
-  int32_t bloom_order[LJ_MAX_DIMS];
   bool bo_default = true;
   for (int i = 0; i < nd; i++) bo_default = bo_default && q->bloom_order[i] < 0;
-  for (int i = 0; i < nd; i++) bloom_order[i] = bo_default ? q->order[i] : q->bloom_order[i];
-  if (!is_perm(bloom_order, nd)) return fail(c, LJ_EINVAL, "bloom_order is not a permutation");
+  for (int i = 0; i < nd; i++) p.bloom_order[i] = bo_default ? q->order[i] : q->bloom_order[i];
+  if (!is_perm(p.bloom_order, nd)) return fail(c, LJ_EINVAL, "bloom_order is not a permutation");
   if (q->n_groups < 1 || q->n_groups > (1ll << 26)) return fail(c, LJ_EINVAL, "n_groups %lld not in [1, 2^26]", (long long)q->n_groups);
   if (q->n_fact > 0) {
     if (!q->fact_measure || ((uintptr_t)q->fact_measure & 15)) return fail(c, LJ_EINVAL, "fact_measure must be a 16-byte aligned device pointer");
@@ -626,45 +660,45 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
       if (!q->fact_fk[d] || ((uintptr_t)q->fact_fk[d] & 15))
         return fail(c, LJ_EINVAL, "fact_fk[%d] must be a 16-byte aligned device pointer", d);
   }
-  HIP_TRY(c, hipSetDevice(c->device));
+  p.nd = nd;
+  return LJ_OK;
+}
 
-  // dimensions: resident, or built inside the call
-  lj_dim* dims[LJ_MAX_DIMS] = {nullptr, nullptr, nullptr, nullptr};
-  lj_dim* owned[LJ_MAX_DIMS];
-  int n_owned = 0;
+// Resident dimensions, or tables allocated (all zero) for a build inside the call.
+static lj_status acquire_dims(lj_ctx* c, const lj_star_desc* q, Plan& p) {
   lj_status s = LJ_OK;
-  for (int d = 0; d < nd && s == LJ_OK; d++) {
+  for (int d = 0; d < p.nd && s == LJ_OK; d++) {
     if (q->resident[d]) {
-      dims[d] = const_cast<lj_dim*>(q->resident[d]);
+      p.dims[d] = const_cast<lj_dim*>(q->resident[d]);
     } else {
-      s = dim_alloc(c, &q->dims[d], &dims[d]);
-      if (s == LJ_OK) owned[n_owned++] = dims[d];
+      s = dim_alloc(c, &q->dims[d], &p.dims[d]);
+      if (s == LJ_OK) p.owned[p.n_owned++] = p.dims[d];
     }
   }
-  auto cleanup = [&]() { dims_free(c, owned, n_owned); };
   if (s != LJ_OK) {
-    cleanup();
-    return s;
+    dims_free(c, p.owned, p.n_owned);
+    p.n_owned = 0;
   }
-  if ((s = ensure_groups(c, q->n_groups)) != LJ_OK) {
-    cleanup();
-    return s;
-  }
+  return s;
+}
 
-  // ---- probe plan: workgroup shape, LDS-resident filters, grid
+// Workgroup shape, LDS-resident filters, stage-0 kernel kind, occupancy.
+static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, int wg, Plan& p) {
+  const int nd = p.nd;
   const bool lip = q->method != LJ_METHOD_NAIVE;
   const bool adaptive = q->method == LJ_METHOD_LIP_ADAPTIVE;
-  ProbeConfig cfg;
-  cfg.wg = c->probe_wg ? c->probe_wg : 1024;
+  ProbeConfig& cfg = p.cfg;
+  cfg.wg = wg ? wg : (c->probe_wg ? c->probe_wg : 1024);
   cfg.vec = c->probe_vec ? c->probe_vec : 1;
   cfg.k0 = S0_GENERIC;
   const int lds_policy = c->lds_policy >= 0 ? c->lds_policy : 0;
-  ProbeArgs a{};
+  ProbeArgs& a = p.a;
+  a = ProbeArgs{};
   int lds_words = 0;
   const int64_t lds_budget_words = ((int64_t)c->max_lds - (int64_t)probe_lds_bytes(cfg, 0) - 16) / 4;
   for (int d = 0; d < nd; d++) {
     ProbeDim& P = a.dim[d];
-    lj_dim* D = dims[d];
+    lj_dim* D = p.dims[d];
     P.fk = q->fact_fk[d];
     P.bloom = D->bloom;
     P.ht = D->ht;
@@ -679,7 +713,7 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   if (lip) {
     // place filters in LDS in load order; policy 1 keeps only the first one
     for (int st = 0; st < nd; st++) {
-      const int d = adaptive ? st : bloom_order[st];
+      const int d = adaptive ? st : p.bloom_order[st];
       ProbeDim& P = a.dim[d];
       if (P.bh.kind == BLOOM_NONE || !P.bloom) continue;
       const int64_t w4 = (P.bloom_words + 3) & ~3ll;
@@ -704,39 +738,15 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   if (!lip) {
     cfg.k0 = S0_PHT;
   } else if (!adaptive) {
-    cfg.k0 = kind_of(bloom_order[0]);
+    cfg.k0 = kind_of(p.bloom_order[0]);
   } else {
     cfg.k0 = kind_of(0);
     for (int d = 1; d < nd; d++)
       if (kind_of(d) != cfg.k0) cfg.k0 = S0_GENERIC;
   }
-  const size_t lds_bytes = probe_lds_bytes(cfg, lds_words);
-  int per_cu = 0;
-  {
-    const auto key = std::make_pair(cfg.wg * 1000 + cfg.vec * 100 + cfg.k0, lds_bytes);
-    auto it = c->occupancy.find(key);
-    if (it == c->occupancy.end()) {
-      HIP_TRY(c, probe_occupancy(cfg, lds_bytes, &per_cu));
-      c->occupancy[key] = per_cu;
-    } else {
-      per_cu = it->second;
-    }
-  }
-  if (per_cu < 1) {
-    cleanup();
-    return fail(c, LJ_EUNSUPPORTED, "probe configuration wg=%d vec=%d with %zu B LDS does not fit", cfg.wg, cfg.vec, lds_bytes);
-  }
-  const int64_t tile_rows = (int64_t)cfg.vec * kSegRows;
-  const int nw = cfg.wg / 64;
-  int64_t grid = (int64_t)c->n_cu * per_cu;
-  const int64_t need = (q->n_fact + tile_rows * nw - 1) / (tile_rows * nw);
-  if (grid > need) grid = need;
-  if (grid < 1) grid = 1;
-  if ((s = ensure_stats(c, (int)grid)) != LJ_OK) {
-    cleanup();
-    return s;
-  }
-
+  p.lds_bytes = probe_lds_bytes(cfg, lds_words);
+  lj_status s = occupancy_of(c, cfg, p.lds_bytes, &p.per_cu);
+  if (s != LJ_OK) return s;
   a.measure = q->fact_measure;
   a.n = q->n_fact;
   a.n_dims = nd;
@@ -745,11 +755,67 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   a.want_product = want_product;
   for (int i = 0; i < LJ_MAX_DIMS; i++) {
     a.order[i] = i < nd ? q->order[i] : i;
-    a.load_order[i] = i < nd ? (lip ? bloom_order[i] : q->order[i]) : i;
+    a.load_order[i] = i < nd ? (lip ? p.bloom_order[i] : q->order[i]) : i;
   }
   a.lds_filter_words = lds_words;
   a.goffset = q->group_offset;
   a.n_groups = q->n_groups;
+  for (int d = 0; d < nd; d++) a.dim_counters[d] = q->resident[d] ? nullptr : p.dims[d]->counters;
+  return LJ_OK;
+}
+
+// Parse one query's result region (header + groups, host copy).
+static lj_status parse_result(lj_ctx* c, const lj_star_desc* q, const Plan& p, const long long* hdr,
+                              const GroupOut* groups, int64_t groups_avail, QueryOut& qo) {
+  const int nd = p.nd;
+  const int64_t n_touched = hdr[0];
+  memcpy(qo.stats, hdr + 1, sizeof qo.stats);
+  qo.groups.assign(groups, groups + std::min<int64_t>(n_touched, groups_avail));
+  std::sort(qo.groups.begin(), qo.groups.end(), [](const GroupOut& x, const GroupOut& y) { return x.g < y.g; });
+  int64_t build_oob = 0;
+  for (int d = 0; d < nd; d++) {
+    if (q->resident[d]) {
+      qo.dim_selected[d] = p.dims[d]->n_selected;
+      qo.dim_oob[d] = p.dims[d]->n_oob;
+    } else {
+      qo.dim_selected[d] = (int64_t)hdr[1 + kStats + 2 * d];
+      qo.dim_oob[d] = (int64_t)hdr[2 + kStats + 2 * d];
+    }
+    build_oob += qo.dim_oob[d];
+  }
+  if (build_oob > 0 || qo.stats[ST_OOB] > 0)
+    return fail(c, LJ_EDOMAIN, "%lld dimension keys and %lld fact keys/groups fell outside their hash domains",
+                (long long)build_oob, (long long)qo.stats[ST_OOB]);
+  return LJ_OK;
+}
+
+// The common path of lj_star_join and lj_hash_join.
+static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, QueryOut& qo) {
+  Plan p;
+  lj_status s = validate_query(c, q, p);
+  if (s != LJ_OK) return s;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if ((s = acquire_dims(c, q, p)) != LJ_OK) return s;
+  auto cleanup = [&]() {
+    dims_free(c, p.owned, p.n_owned);
+    p.n_owned = 0;
+  };
+  if ((s = ensure_groups(c, q->n_groups)) != LJ_OK || (s = plan_probe(c, q, want_product, 0, p)) != LJ_OK) {
+    cleanup();
+    return s;
+  }
+  const ProbeConfig& cfg = p.cfg;
+  ProbeArgs& a = p.a;
+  const int64_t tile_rows = (int64_t)cfg.vec * kSegRows;
+  const int nw = cfg.wg / 64;
+  int64_t grid = (int64_t)c->n_cu * p.per_cu;
+  const int64_t need = (q->n_fact + tile_rows * nw - 1) / (tile_rows * nw);
+  if (grid > need) grid = need;
+  if (grid < 1) grid = 1;
+  if ((s = ensure_stats(c, (int)grid)) != LJ_OK) {
+    cleanup();
+    return s;
+  }
   a.gsum = c->gsum;
   a.gcount = c->gcount;
   a.gprod = c->gprod;
@@ -758,25 +824,23 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   a.n_touched = c->n_touched;
   a.touched_cap = c->touched_cap;
   a.stats = c->stats;
-
   a.done = c->n_touched + 32;
   a.out_hdr = c->out_hdr;
   a.out = c->out;
   a.out_cap = c->touched_cap;
-  for (int d = 0; d < nd; d++) a.dim_counters[d] = q->resident[d] ? nullptr : dims[d]->counters;
 
   // ---- two launches: the build of the query's own dimensions, then the probe
   // (which also collects the result groups in its last workgroup)
-  const bool built_here = n_owned > 0;
+  const bool built_here = p.n_owned > 0;
   HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
   if (built_here) {
-    if ((s = dims_launch(c, owned, n_owned, c->out_hdr + kClockSlot)) != LJ_OK) {
+    if ((s = dims_launch(c, p.owned, p.n_owned, c->out_hdr + kClockSlot)) != LJ_OK) {
       cleanup();
       return s;
     }
   }
   HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
-  HIP_TRY(c, launch_probe(cfg, a, (int)grid, lds_bytes, c->stream));
+  HIP_TRY(c, launch_probe(cfg, a, (int)grid, p.lds_bytes, c->stream));
   HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
 
   // ---- results
@@ -788,54 +852,30 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
   // the built tables go back to the zero pool now: their clears queue behind the copy
   cleanup();
-  n_owned = 0;
   HIP_TRY(c, hipEventSynchronize(c->ev[2]));
   const int64_t n_touched = hdr[0];
-  memcpy(qo.stats, hdr + 1, sizeof qo.stats);
-  unsigned long long cnt_h[kMaxDims][2];
-  for (int d = 0; d < nd; d++) {
-    cnt_h[d][0] = (unsigned long long)hdr[1 + kStats + 2 * d];
-    cnt_h[d][1] = (unsigned long long)hdr[2 + kStats + 2 * d];
-  }
-  if (n_touched > (int64_t)c->touched_cap) {
-    cleanup();
+  if (n_touched > (int64_t)c->touched_cap)
     return fail(c, LJ_EUNSUPPORTED, "%lld groups were hit; at most %u can be listed", (long long)n_touched, c->touched_cap);
-  }
+  std::vector<GroupOut> big;
+  const GroupOut* groups = hg;
   if (n_touched > first) {
-    const int64_t upto = std::min<int64_t>(n_touched, kHostGroups);
-    HIP_TRY(c, hipMemcpyAsync(hg + first, c->out + first, sizeof(GroupOut) * (size_t)(upto - first),
-                              hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (n_touched <= kHostGroups) {
+      HIP_TRY(c, hipMemcpyAsync(hg + first, c->out + first, sizeof(GroupOut) * (size_t)(n_touched - first),
+                                hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+    } else {
+      big.resize((size_t)n_touched);
+      HIP_TRY(c, hipMemcpy(big.data(), c->out, sizeof(GroupOut) * (size_t)n_touched, hipMemcpyDeviceToHost));
+      groups = big.data();
+    }
   }
-  qo.groups.assign(hg, hg + std::min<int64_t>(n_touched, kHostGroups));
-  if (n_touched > kHostGroups) {
-    qo.groups.resize((size_t)n_touched);
-    HIP_TRY(c, hipMemcpy(qo.groups.data() + kHostGroups, c->out + kHostGroups,
-                         sizeof(GroupOut) * (size_t)(n_touched - kHostGroups), hipMemcpyDeviceToHost));
-  }
-  std::sort(qo.groups.begin(), qo.groups.end(), [](const GroupOut& x, const GroupOut& y) { return x.g < y.g; });
   // phases from HIP events on the query's stream: [build launch] [probe launch]
   qo.t_fill = 0.f;
   HIP_TRY(c, hipEventElapsedTime(&qo.t_build, c->ev[0], c->ev[3]));
   HIP_TRY(c, hipEventElapsedTime(&qo.t_probe, c->ev[3], c->ev[1]));
   HIP_TRY(c, hipEventElapsedTime(&qo.t_total, c->ev[0], c->ev[1]));
   if (!built_here) qo.t_build = 0.f;
-  int64_t build_oob = 0;
-  for (int d = 0; d < nd; d++) {
-    if (q->resident[d]) {
-      qo.dim_selected[d] = dims[d]->n_selected;
-      qo.dim_oob[d] = dims[d]->n_oob;
-    } else {
-      qo.dim_selected[d] = (int64_t)cnt_h[d][0];
-      qo.dim_oob[d] = (int64_t)cnt_h[d][1];
-    }
-    build_oob += qo.dim_oob[d];
-  }
-  cleanup();
-  if (build_oob > 0 || qo.stats[ST_OOB] > 0)
-    return fail(c, LJ_EDOMAIN, "%lld dimension keys and %lld fact keys/groups fell outside their hash domains",
-                (long long)build_oob, (long long)qo.stats[ST_OOB]);
-  return LJ_OK;
+  return parse_result(c, q, p, hdr, groups, n_touched, qo);
 }
 
 static void fill_stats(const QueryOut& qo, int nd, lj_probe_stats* ps) {
@@ -851,11 +891,7 @@ static void fill_stats(const QueryOut& qo, int nd, lj_probe_stats* ps) {
   for (int s = 0; s < LJ_MAX_DIMS; s++) ps->order_used[s] = s < nd ? (int32_t)((qo.stats[ST_ORDER] >> (8 * s)) & 0xff) : -1;
 }
 
-extern "C" lj_status lj_star_join(lj_ctx* c, const lj_star_desc* q, lj_star_result* out) {
-  if (!c || !q || !out) return fail(c, LJ_EINVAL, "NULL argument");
-  QueryOut qo;
-  lj_status s = run_query(c, q, 0, qo);
-  if (s != LJ_OK && s != LJ_EDOMAIN) return s;
+static void to_star_result(const lj_star_desc* q, const QueryOut& qo, lj_star_result* out) {
   out->n_groups = (int64_t)qo.groups.size();
   out->n_rows = 0;
   out->sum_total = 0;
@@ -878,6 +914,14 @@ extern "C" lj_status lj_star_join(lj_ctx* c, const lj_star_desc* q, lj_star_resu
   out->time_probe_ms = qo.t_probe;
   out->reserved0 = 0;
   fill_stats(qo, q->n_dims, &out->stats);
+}
+
+extern "C" lj_status lj_star_join(lj_ctx* c, const lj_star_desc* q, lj_star_result* out) {
+  if (!c || !q || !out) return fail(c, LJ_EINVAL, "NULL argument");
+  QueryOut qo;
+  lj_status s = run_query(c, q, 0, qo);
+  if (s != LJ_OK && s != LJ_EDOMAIN) return s;
+  to_star_result(q, qo, out);
   return s;
 }
 
@@ -960,5 +1004,453 @@ extern "C" lj_status lj_gen_shuffle_host(uint64_t seed, uint64_t stream, int64_t
     out[i] = out[j];
     out[j] = t;
   }
+  return LJ_OK;
+}
+
+// ------------------------------------------------------------------- batches --
+// n queries in one build launch and one probe launch per stage-0 kernel kind;
+// query q's workgroups are [wg_begin[q], wg_begin[q+1]) of its kind's grid.
+namespace {
+struct BatchQuery {
+  lj_star_desc desc;
+  Plan p;
+  int64_t grid = 0;
+  int64_t tcap = 0;
+  size_t zero_off = 0;   // gsum | gcount | gprod | n_touched+done (kept zero between runs)
+  size_t plain_off = 0;  // gpay | touched | stats
+  size_t out_off = 0;    // header | groups
+  int group = 0;
+};
+struct BatchKind {
+  ProbeConfig cfg{};
+  size_t lds = 0;
+  int per_cu = 0;
+  std::vector<int> qs;
+  int64_t grid = 0;
+  ProbeArgs* d_qs = nullptr;
+  uint32_t* d_wgb = nullptr;
+};
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+struct lj_batch {
+  lj_ctx* c = nullptr;
+  lj_batch_opts opts{};
+  std::vector<BatchQuery> q;
+  std::vector<BatchKind> kinds;
+  BuildDim* d_build = nullptr;
+  int n_build = 0;
+  int64_t build_blocks = 0;
+  std::vector<FillArgs> clears;   // the built tables, cleared before every run but the first
+  char* zero_buf = nullptr;
+  char* plain_buf = nullptr;
+  char* out_dev = nullptr;
+  char* out_host = nullptr;
+  size_t out_bytes = 0;
+  void* args_buf = nullptr;       // ProbeArgs[] and wg_begin[] of every kind, BuildDim[]
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int runs = 0;
+};
+
+static void batch_free(lj_batch* b) {
+  if (!b) return;
+  lj_ctx* c = b->c;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto& bq : b->q) dims_free(c, bq.p.owned, bq.p.n_owned);
+  if (b->zero_buf) {
+    FillArgs f{};
+    dev_release_zero(c, b->zero_buf, f);
+    (void)launch_fill(f, c->stream);
+  }
+  dev_release(c, b->plain_buf);
+  dev_release(c, b->out_dev);
+  dev_release(c, b->args_buf);
+  if (b->out_host) (void)hipHostFree(b->out_host);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" lj_status lj_batch_create(lj_ctx* c, const lj_star_desc* descs, int32_t n, const lj_batch_opts* opts,
+                                     lj_batch** out) {
+  if (!c || !out || (n > 0 && !descs)) return fail(c, LJ_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (n < 1 || n > (1 << 16)) return fail(c, LJ_EINVAL, "batch of %d queries (1..65536)", n);
+  HIP_TRY(c, hipSetDevice(c->device));
+  lj_batch* b = new lj_batch();
+  b->c = c;
+  if (opts) b->opts = *opts;
+  if (b->opts.groups_cap <= 0) b->opts.groups_cap = 64;
+  const int wg = b->opts.probe_wg;
+  if (wg != 0 && wg != 256 && wg != 512 && wg != 1024) {
+    delete b;
+    return fail(c, LJ_EINVAL, "probe_wg %d (0, 256, 512 or 1024)", wg);
+  }
+  auto bail = [&](lj_status s) {
+    batch_free(b);
+    return s;
+  };
+  b->q.resize((size_t)n);
+  // plan every query; group them by stage-0 kernel kind
+  for (int i = 0; i < n; i++) {
+    BatchQuery& bq = b->q[(size_t)i];
+    bq.desc = descs[i];
+    lj_status s = validate_query(c, &bq.desc, bq.p);
+    if (s == LJ_OK) s = acquire_dims(c, &bq.desc, bq.p);
+    if (s == LJ_OK) s = plan_probe(c, &bq.desc, 0, wg, bq.p);
+    if (s != LJ_OK) {
+      c->err = "query " + std::to_string(i) + ": " + c->err;
+      return bail(s);
+    }
+    int k = -1;
+    for (size_t j = 0; j < b->kinds.size(); j++)
+      if (b->kinds[j].cfg.wg == bq.p.cfg.wg && b->kinds[j].cfg.vec == bq.p.cfg.vec && b->kinds[j].cfg.k0 == bq.p.cfg.k0)
+        k = (int)j;
+    if (k < 0) {
+      b->kinds.push_back(BatchKind{});
+      k = (int)b->kinds.size() - 1;
+      b->kinds[(size_t)k].cfg = bq.p.cfg;
+    }
+    BatchKind& K = b->kinds[(size_t)k];
+    K.lds = std::max(K.lds, bq.p.lds_bytes);
+    K.qs.push_back(i);
+    bq.group = k;
+  }
+  // workgroups: about four per resident slot of the kind, in proportion to the
+  // fact rows, at least one per query and no more than it has tiles for
+  for (BatchKind& K : b->kinds) {
+    lj_status s = occupancy_of(c, K.cfg, K.lds, &K.per_cu);
+    if (s != LJ_OK) return bail(s);
+    const int64_t slots = (int64_t)c->n_cu * K.per_cu;
+    int64_t rows = 0;
+    for (int i : K.qs) rows += b->q[(size_t)i].desc.n_fact;
+    const int64_t tile_rows = (int64_t)K.cfg.vec * kSegRows * (K.cfg.wg / 64);
+    for (int i : K.qs) {
+      BatchQuery& bq = b->q[(size_t)i];
+      const int64_t need = std::max<int64_t>(1, (bq.desc.n_fact + tile_rows - 1) / tile_rows);
+      int64_t g = rows > 0 ? (int64_t)((double)bq.desc.n_fact / (double)rows * (double)(4 * slots) + 0.5) : 1;
+      bq.grid = std::min(std::max<int64_t>(g, 1), need);
+      K.grid += bq.grid;
+    }
+    if (K.grid >= (1ll << 31)) return bail(fail(c, LJ_EUNSUPPORTED, "batch grid too large"));
+  }
+  // per-query buffers, carved from three allocations
+  size_t zero_bytes = 0, plain_bytes = 0, out_bytes = 0;
+  for (BatchQuery& bq : b->q) {
+    const int64_t ng = bq.desc.n_groups;
+    bq.tcap = std::min<int64_t>(ng, b->opts.groups_cap);
+    bq.zero_off = zero_bytes;
+    zero_bytes += align256((size_t)ng * 8) * 3 + 256;
+    bq.plain_off = plain_bytes;
+    plain_bytes += align256((size_t)ng * kMaxDims * 4) + align256((size_t)bq.tcap * 4) +
+                   align256((size_t)bq.grid * kStats * 8);
+    bq.out_off = out_bytes;
+    out_bytes += align256(kHdrBytes + sizeof(GroupOut) * (size_t)bq.tcap);
+  }
+  lj_status s;
+  void* zp = nullptr;
+  void* pp = nullptr;
+  void* op = nullptr;
+  if ((s = dev_alloc_zero(c, zero_bytes, &zp)) != LJ_OK) return bail(s);
+  b->zero_buf = static_cast<char*>(zp);
+  if ((s = dev_alloc(c, plain_bytes, &pp)) != LJ_OK) return bail(s);
+  b->plain_buf = static_cast<char*>(pp);
+  if ((s = dev_alloc(c, out_bytes, &op)) != LJ_OK) return bail(s);
+  b->out_dev = static_cast<char*>(op);
+  b->out_bytes = out_bytes;
+  HIP_TRY(c, hipHostMalloc((void**)&b->out_host, out_bytes, hipHostMallocDefault));
+  // the argument arrays of every kind and the build descriptors
+  size_t args_bytes = 0;
+  std::vector<size_t> qs_off(b->kinds.size()), wgb_off(b->kinds.size());
+  for (size_t k = 0; k < b->kinds.size(); k++) {
+    qs_off[k] = args_bytes;
+    args_bytes += align256(sizeof(ProbeArgs) * b->kinds[k].qs.size());
+    wgb_off[k] = args_bytes;
+    args_bytes += align256(sizeof(uint32_t) * (b->kinds[k].qs.size() + 1));
+  }
+  std::vector<BuildDim> bds;
+  for (BatchQuery& bq : b->q)
+    for (int i = 0; i < bq.p.n_owned; i++) {
+      BuildDim bd{};
+      set_build_dim(bq.p.owned[i], bd);
+      bd.block_begin = b->build_blocks;
+      b->build_blocks += build_blocks(bd.n);
+      bds.push_back(bd);
+    }
+  b->n_build = (int)bds.size();
+  const size_t build_off = args_bytes;
+  args_bytes += align256(sizeof(BuildDim) * std::max<size_t>(bds.size(), 1));
+  if ((s = dev_alloc(c, args_bytes, &b->args_buf)) != LJ_OK) return bail(s);
+  char* args = static_cast<char*>(b->args_buf);
+  std::vector<char> host_args(args_bytes, 0);
+  for (size_t k = 0; k < b->kinds.size(); k++) {
+    BatchKind& K = b->kinds[k];
+    K.d_qs = reinterpret_cast<ProbeArgs*>(args + qs_off[k]);
+    K.d_wgb = reinterpret_cast<uint32_t*>(args + wgb_off[k]);
+    ProbeArgs* hq = reinterpret_cast<ProbeArgs*>(host_args.data() + qs_off[k]);
+    uint32_t* hw = reinterpret_cast<uint32_t*>(host_args.data() + wgb_off[k]);
+    uint32_t begin = 0;
+    for (size_t j = 0; j < K.qs.size(); j++) {
+      BatchQuery& bq = b->q[(size_t)K.qs[j]];
+      ProbeArgs a = bq.p.a;
+      const int64_t ng = bq.desc.n_groups;
+      char* z = b->zero_buf + bq.zero_off;
+      a.gsum = reinterpret_cast<unsigned long long*>(z);
+      a.gcount = reinterpret_cast<unsigned long long*>(z + align256((size_t)ng * 8));
+      a.gprod = reinterpret_cast<unsigned long long*>(z + 2 * align256((size_t)ng * 8));
+      a.n_touched = reinterpret_cast<uint32_t*>(z + 3 * align256((size_t)ng * 8));
+      a.done = a.n_touched + 32;
+      char* pl = b->plain_buf + bq.plain_off;
+      a.gpay = reinterpret_cast<int32_t*>(pl);
+      a.touched = reinterpret_cast<uint32_t*>(pl + align256((size_t)ng * kMaxDims * 4));
+      a.stats = reinterpret_cast<long long*>(pl + align256((size_t)ng * kMaxDims * 4) + align256((size_t)bq.tcap * 4));
+      a.touched_cap = (uint32_t)bq.tcap;
+      a.out_hdr = reinterpret_cast<long long*>(b->out_dev + bq.out_off);
+      a.out = reinterpret_cast<GroupOut*>(b->out_dev + bq.out_off + kHdrBytes);
+      a.out_cap = bq.tcap;
+      hq[j] = a;
+      hw[j] = begin;
+      begin += (uint32_t)bq.grid;
+    }
+    hw[K.qs.size()] = begin;
+  }
+  if (!bds.empty()) memcpy(host_args.data() + build_off, bds.data(), sizeof(BuildDim) * bds.size());
+  b->d_build = reinterpret_cast<BuildDim*>(args + build_off);
+  HIP_TRY(c, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIP_TRY(c, hipEventCreate(&b->e0));
+  HIP_TRY(c, hipEventCreate(&b->e1));
+  // after everything queued on the context's stream (zero-pool clears)
+  HIP_TRY(c, hipEventRecord(b->e0, c->stream));
+  HIP_TRY(c, hipStreamWaitEvent(b->stream, b->e0, 0));
+  HIP_TRY(c, hipMemcpyAsync(args, host_args.data(), args_bytes, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(c, hipStreamSynchronize(b->stream));
+  // the tables to clear before a repeated run
+  FillArgs f{};
+  for (BatchQuery& bq : b->q)
+    for (int i = 0; i < bq.p.n_owned; i++) {
+      lj_dim* D = bq.p.owned[i];
+      void* ps[3] = {D->ht, D->bloom, D->counters};
+      int64_t ws[3] = {D->ht_words, D->bloom_words, kCounterWords};
+      for (int t = 0; t < 3; t++) {
+        if (!ps[t] || ws[t] <= 0) continue;
+        if (f.n == kMaxFill) {
+          b->clears.push_back(f);
+          f = FillArgs{};
+        }
+        add_fill(f, ps[t], ws[t]);
+      }
+    }
+  if (f.n) b->clears.push_back(f);
+  *out = b;
+  return LJ_OK;
+}
+
+// The launches of one run (recorded once when use_graph).
+static lj_status batch_enqueue(lj_batch* b, bool clear) {
+  lj_ctx* c = b->c;
+  if (clear)
+    for (const FillArgs& f : b->clears) HIP_TRY(c, launch_fill(f, b->stream));
+  HIP_TRY(c, launch_build_multi(b->d_build, b->n_build, b->build_blocks, b->stream));
+  for (const BatchKind& K : b->kinds)
+    HIP_TRY(c, launch_probe_multi(K.cfg, K.d_qs, K.d_wgb, (int)K.qs.size(), (int)K.grid, K.lds, b->stream));
+  HIP_TRY(c, hipMemcpyAsync(b->out_host, b->out_dev, b->out_bytes, hipMemcpyDeviceToHost, b->stream));
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_batch_run(lj_batch* b, lj_star_result* results) {
+  if (!b) return fail(nullptr, LJ_EINVAL, "NULL batch");
+  lj_ctx* c = b->c;
+  if (!results) return fail(c, LJ_EINVAL, "NULL results");
+  HIP_TRY(c, hipSetDevice(c->device));
+  lj_status s;
+  if (b->opts.use_graph) {
+    if (!b->exec) {
+      HIP_TRY(c, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      s = batch_enqueue(b, true);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (s != LJ_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return s;
+      }
+      HIP_TRY(c, e);
+      b->graph = g;
+      HIP_TRY(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(c, hipEventRecord(b->e0, b->stream));
+    HIP_TRY(c, hipGraphLaunch(b->exec, b->stream));
+    HIP_TRY(c, hipEventRecord(b->e1, b->stream));
+  } else {
+    HIP_TRY(c, hipEventRecord(b->e0, b->stream));
+    if ((s = batch_enqueue(b, b->runs > 0)) != LJ_OK) return s;
+    HIP_TRY(c, hipEventRecord(b->e1, b->stream));
+  }
+  HIP_TRY(c, hipEventSynchronize(b->e1));
+  b->runs++;
+  float ms = 0.f;
+  HIP_TRY(c, hipEventElapsedTime(&ms, b->e0, b->e1));
+  lj_status first_err = LJ_OK;
+  for (size_t i = 0; i < b->q.size(); i++) {
+    BatchQuery& bq = b->q[i];
+    const long long* hdr = reinterpret_cast<const long long*>(b->out_host + bq.out_off);
+    const GroupOut* groups = reinterpret_cast<const GroupOut*>(b->out_host + bq.out_off + kHdrBytes);
+    QueryOut qo;
+    s = parse_result(c, &bq.desc, bq.p, hdr, groups, bq.tcap, qo);
+    if (s == LJ_OK && hdr[0] > bq.tcap)
+      s = fail(c, LJ_EUNSUPPORTED, "query %zu hit %lld groups; groups_cap is %lld", i, hdr[0], (long long)bq.tcap);
+    qo.t_total = qo.t_probe = ms;
+    qo.t_build = qo.t_fill = 0.f;
+    to_star_result(&bq.desc, qo, &results[i]);
+    if (s != LJ_OK && first_err == LJ_OK) first_err = s;
+  }
+  return first_err;
+}
+
+extern "C" void lj_batch_destroy(lj_batch* b) { batch_free(b); }
+
+// ------------------------------------------------------------ Bloom autotuner --
+// False-positive rate of a filter of m bits holding n keys.
+static double fpr_of(int hash, int layout, int k, double n, double m, double dim_rows) {
+  if (n <= 0) return 0.0;
+  if (hash == LJ_BLOOM_MOD) {
+    // bit (key - min) % m over keys dense in [min, min + dim_rows): exact when
+    // m covers the range; else a bit is set if any of its ~dim_rows/m keys is
+    if (m >= dim_rows) return 0.0;
+    const double sel = std::min(1.0, n / dim_rows);
+    return 1.0 - std::pow(1.0 - sel, dim_rows / m);
+  }
+  if (layout == LJ_BLOOM_STANDARD) return std::pow(1.0 - std::exp(-(double)k * n / m), (double)k);
+  // blocked: k bits in one 32-bit word of m / 32; a word holds Poisson(lambda) keys
+  const double words = m / 32.0, lambda = n / words;
+  double p = std::exp(-lambda), acc = 0.0;
+  for (int j = 0; j < 512; j++) {
+    if (j > 0) p *= lambda / j;
+    const double filled = 1.0 - std::pow(1.0 - 1.0 / 32.0, (double)k * j);
+    acc += p * std::pow(filled, (double)k);
+    if (j > lambda && p < 1e-12) break;
+  }
+  return acc;
+}
+
+extern "C" lj_status lj_tune_bloom(lj_ctx* c, int64_t n_selected, int64_t n_dim_rows, double fact_pass,
+                                   lj_bloom_choice* out) {
+  if (!c || !out) return fail(c, LJ_EINVAL, "NULL argument");
+  if (n_selected < 0 || n_dim_rows < 1 || n_selected > n_dim_rows || !(fact_pass >= 0.0 && fact_pass <= 1.0))
+    return fail(c, LJ_EINVAL, "n_selected %lld, n_dim_rows %lld, fact_pass %g", (long long)n_selected,
+                (long long)n_dim_rows, fact_pass);
+  ProbeConfig cfg{1024, 1, S0_MULT_LDS};
+  const int64_t lds_bits = ((int64_t)c->max_lds - (int64_t)probe_lds_bytes(cfg, 0) - 16) * 8;
+  lj_bloom_choice best{};
+  best.cost = 1e300;
+  struct Cand { int hash, layout, k; };
+  const Cand cands[] = {{LJ_BLOOM_MOD, LJ_BLOOM_STANDARD, 1},  {LJ_BLOOM_MULT, LJ_BLOOM_STANDARD, 1},
+                        {LJ_BLOOM_MULT, LJ_BLOOM_STANDARD, 2}, {LJ_BLOOM_MULT, LJ_BLOOM_STANDARD, 3},
+                        {LJ_BLOOM_MULT, LJ_BLOOM_BLOCKED, 2},  {LJ_BLOOM_MULT, LJ_BLOOM_BLOCKED, 3}};
+  for (int lg = 16; lg <= 26; lg++) {
+    const double m = std::ldexp(1.0, lg);
+    for (const Cand& cd : cands) {
+      const bool in_lds = (int64_t)m <= lds_bits;
+      const double fpr = fpr_of(cd.hash, cd.layout, cd.k, (double)n_selected, m, (double)n_dim_rows);
+      double scan = in_lds ? 1.0 : c->tune_scan_gl;
+      if (cd.layout == LJ_BLOOM_STANDARD && cd.k > 1) scan += (cd.k - 1) * c->tune_probe_extra * (in_lds ? 1.0 : c->tune_scan_gl);
+      const double cost = scan + (fact_pass + (1.0 - fact_pass) * fpr) * c->tune_surv;
+      if (cost < best.cost * (1.0 - 1e-9)) {
+        best.bloom_hash = cd.hash;
+        best.bloom_layout = cd.layout;
+        best.bloom_k = cd.k;
+        best.in_lds = in_lds ? 1 : 0;
+        best.bloom_bits = (int64_t)m;
+        best.fpr = fpr;
+        best.cost = cost;
+      }
+    }
+  }
+  *out = best;
+  return LJ_OK;
+}
+
+// Calibration: a dimension of 2^22 dense keys and a fact table of 2^24 rows
+// (device-generated), three LIP joins: a filter in LDS that nothing passes,
+// a filter in L2 that nothing passes, and an exact filter that 10 % pass.
+extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
+  if (!c) return fail(c, LJ_EINVAL, "NULL context");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const int64_t nd = 1 << 22, nf = 1 << 24;
+  int32_t *keys = nullptr, *attr = nullptr, *fk = nullptr, *meas = nullptr;
+  lj_status s;
+  if ((s = dev_alloc_t(c, nd, &keys)) != LJ_OK) return s;
+  auto release = [&]() {
+    (void)hipStreamSynchronize(c->stream);
+    dev_release(c, keys);
+    dev_release(c, attr);
+    dev_release(c, fk);
+    dev_release(c, meas);
+  };
+  if ((s = dev_alloc_t(c, nd, &attr)) != LJ_OK || (s = dev_alloc_t(c, nf, &fk)) != LJ_OK ||
+      (s = dev_alloc_t(c, nf, &meas)) != LJ_OK) {
+    release();
+    return s;
+  }
+  const uint64_t seed = 0x1A7E5EED7Full;
+  if (launch_gen(LJ_GEN_SEQUENCE, mix64(seed), 0, 0, 0, nd, keys, c->stream) != hipSuccess ||
+      launch_gen(LJ_GEN_UNIFORM, stream_key(seed, 1), 0, 9999, 0, nd, attr, c->stream) != hipSuccess ||
+      launch_gen(LJ_GEN_UNIFORM, stream_key(seed, 2), 0, nd - 1, 0, nf, fk, c->stream) != hipSuccess ||
+      launch_gen(LJ_GEN_UNIFORM, stream_key(seed, 3), 1, 10000, 0, nf, meas, c->stream) != hipSuccess) {
+    release();
+    return fail(c, LJ_EHIP, "generator launch failed");
+  }
+  auto run = [&](int hash, int64_t bits, int32_t below, float* ms) -> lj_status {
+    lj_star_desc q{};
+    q.fact_fk[0] = fk;
+    q.fact_measure = meas;
+    q.n_fact = nf;
+    q.n_dims = 1;
+    q.method = LJ_METHOD_LIP;
+    for (int i = 0; i < LJ_MAX_DIMS; i++) {
+      q.order[i] = i;
+      q.bloom_order[i] = -1;
+    }
+    lj_dim_desc& d = q.dims[0];
+    d.keys = keys;
+    d.attr = attr;
+    d.n_rows = nd;
+    d.pred.n_steps = 1;
+    d.pred.steps[0].comb = LJ_COMB_SET;
+    d.pred.steps[0].cmp = LJ_CMP_LT;
+    d.pred.steps[0].value = below;
+    d.ht_kind = LJ_HT_PHT2;
+    d.ht_len = nd;
+    d.bloom_hash = hash;
+    d.bloom_k = 1;
+    d.bloom_bits = bits;
+    q.n_groups = 1;
+    float best = 1e30f;
+    for (int r = 0; r < 4; r++) {
+      QueryOut qo;
+      lj_status st = run_query(c, &q, 0, qo);
+      if (st != LJ_OK) return st;
+      if (r > 0) best = std::min(best, qo.t_probe);
+    }
+    *ms = best;
+    return LJ_OK;
+  };
+  float t_lds = 0, t_gl = 0, t_pass = 0;
+  if ((s = run(LJ_BLOOM_MULT, 1 << 20, 0, &t_lds)) != LJ_OK || (s = run(LJ_BLOOM_MOD, 1 << 25, 0, &t_gl)) != LJ_OK ||
+      (s = run(LJ_BLOOM_MOD, nd, 1000, &t_pass)) != LJ_OK) {
+    release();
+    return s;
+  }
+  release();
+  if (!(t_lds > 0 && t_gl > 0 && t_pass > 0)) return fail(c, LJ_EHIP, "calibration timings are not positive");
+  c->tune_scan_gl = std::max(1.0, (double)t_gl / t_lds);
+  // t_pass scans an L2 filter (2^22 bits = 512 KB) and passes 10 % of the rows
+  c->tune_surv = std::max(1.0, ((double)t_pass - t_gl) / 0.1 / t_lds);
+  c->tune_calibrated = 1;
   return LJ_OK;
 }

@@ -163,7 +163,8 @@ struct ProbeConfig {
 typedef void (*ProbeKernelFn)(ProbeArgs);
 struct ProbeEntry {
   int wg, vec, k0;
-  const void* fn;
+  const void* fn;        // probe_kernel (one query)
+  const void* fn_multi;  // probe_multi_kernel (a batch of queries)
 };
 int probe_entries_pht(const ProbeEntry** out);
 int probe_entries_modp2(const ProbeEntry** out);
@@ -173,6 +174,12 @@ int probe_entries_generic(const ProbeEntry** out);
 
 size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words);
 hipError_t probe_occupancy(const ProbeConfig& c, size_t lds_bytes, int* blocks_per_cu);
+// A batch: qs[n_q] and wg_begin[n_q + 1] in device memory; grid == wg_begin[n_q].
+hipError_t launch_probe_multi(const ProbeConfig& c, const ProbeArgs* qs, const uint32_t* wg_begin, int n_q, int grid,
+                              size_t lds_bytes, hipStream_t s);
+// Every dimension of dims[0..n) (device memory; block_begin set): total_blocks workgroups.
+hipError_t launch_build_multi(const BuildDim* dims, int n, int64_t total_blocks, hipStream_t s);
+int64_t build_blocks(int64_t rows);
 hipError_t launch_probe(const ProbeConfig& c, const ProbeArgs& a, int grid, size_t lds_bytes, hipStream_t s);
 
 }  // namespace lj

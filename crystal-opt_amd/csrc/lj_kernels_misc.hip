@@ -58,22 +58,12 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
 constexpr int kBuildThreads = 256;
 constexpr int kBuildRows = 4;
 
-__global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs args) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  (void)args;
-  const BuildArgs& a = *(const BuildArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
-#else
-  const BuildArgs& a = args;
-#endif
+// Workgroup `blk` of dimension B: rows [blk * 1024, (blk + 1) * 1024).
+__device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
   __shared__ unsigned int cnt[2];
-  if (a.clock && blockIdx.x == 0 && threadIdx.x == 0) *a.clock = (long long)wall_clock64();
-  int d = 0;
-  for (int k = 1; k < a.n_dims; k++)
-    if ((int64_t)blockIdx.x >= a.dim[k].block_begin) d = k;
-  const BuildDim& B = a.dim[d];
   if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t r0 = (((int64_t)blockIdx.x - B.block_begin) * kBuildThreads + threadIdx.x) * kBuildRows;
+  const int64_t r0 = (blk * kBuildThreads + threadIdx.x) * kBuildRows;
   uint32_t nsel = 0, noob = 0;
 #pragma unroll
   for (int j = 0; j < kBuildRows; j++) {
@@ -106,6 +96,42 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs args) {
     if (cnt[0]) atomicAdd(&c[0], (unsigned long long)cnt[0]);
     if (cnt[1]) atomicAdd(&c[1], (unsigned long long)cnt[1]);
   }
+}
+
+__global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs args) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)args;
+  const BuildArgs& a = *(const BuildArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
+#else
+  const BuildArgs& a = args;
+#endif
+  if (a.clock && blockIdx.x == 0 && threadIdx.x == 0) *a.clock = (long long)wall_clock64();
+  int d = 0;
+  for (int k = 1; k < a.n_dims; k++)
+    if ((int64_t)blockIdx.x >= a.dim[k].block_begin) d = k;
+  build_block(a.dim[d], (int64_t)blockIdx.x - a.dim[d].block_begin);
+}
+
+// The dimensions of a batch (descriptors in device memory).
+__global__ __launch_bounds__(kBuildThreads) void build_multi_kernel(const BuildDim* dims, int32_t n) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {  // last dimension with block_begin <= blockIdx.x (uniform)
+    const int mid = (lo + hi + 1) >> 1;
+    if (dims[mid].block_begin <= (int64_t)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  build_block(dims[lo], (int64_t)blockIdx.x - dims[lo].block_begin);
+}
+
+int64_t build_blocks(int64_t rows) {
+  constexpr int64_t rows_per_block = (int64_t)kBuildThreads * kBuildRows;
+  return (rows + rows_per_block - 1) / rows_per_block;
+}
+
+hipError_t launch_build_multi(const BuildDim* dims, int n, int64_t total_blocks, hipStream_t s) {
+  if (n <= 0 || total_blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(build_multi_kernel, dim3((unsigned)total_blocks), dim3(kBuildThreads), 0, s, dims, (int32_t)n);
+  return hipGetLastError();
 }
 
 // Assigns the workgroup ranges (block_begin) and launches.

@@ -15,24 +15,46 @@ size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words) {
   return off;
 }
 
-static const void* find_probe(const ProbeConfig& c) {
+static const ProbeEntry* find_entry(const ProbeConfig& c) {
   int (*tables[])(const ProbeEntry**) = {probe_entries_pht, probe_entries_modp2, probe_entries_mod,
                                          probe_entries_mult, probe_entries_generic};
   for (auto t : tables) {
     const ProbeEntry* e = nullptr;
     const int n = t(&e);
     for (int i = 0; i < n; i++)
-      if (e[i].wg == c.wg && e[i].vec == c.vec && e[i].k0 == c.k0) return e[i].fn;
+      if (e[i].wg == c.wg && e[i].vec == c.vec && e[i].k0 == c.k0) return &e[i];
   }
   return nullptr;
 }
 
+static const void* find_probe(const ProbeConfig& c) {
+  const ProbeEntry* e = find_entry(c);
+  return e ? e->fn : nullptr;
+}
+
+// Also raises both kernels' dynamic LDS limit to the device maximum (a
+// launch with less LDS is unaffected; the limit never has to come down).
 hipError_t probe_occupancy(const ProbeConfig& c, size_t lds_bytes, int* blocks_per_cu) {
-  const void* fn = find_probe(c);
-  if (!fn) return hipErrorInvalidValue;
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  const ProbeEntry* en = find_entry(c);
+  if (!en) return hipErrorInvalidValue;
+  int dev = 0, max_lds = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
   if (e != hipSuccess) return e;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, c.wg, lds_bytes);
+  if ((size_t)max_lds < lds_bytes) return hipErrorInvalidValue;
+  e = hipFuncSetAttribute(en->fn, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(en->fn_multi, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+  if (e != hipSuccess) return e;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, en->fn, c.wg, lds_bytes);
+}
+
+hipError_t launch_probe_multi(const ProbeConfig& c, const ProbeArgs* qs, const uint32_t* wg_begin, int n_q, int grid,
+                              size_t lds_bytes, hipStream_t s) {
+  const ProbeEntry* en = find_entry(c);
+  if (!en) return hipErrorInvalidValue;
+  void* kargs[] = {(void*)&qs, (void*)&wg_begin, (void*)&n_q};
+  return hipLaunchKernel(en->fn_multi, dim3(grid), dim3(c.wg), kargs, lds_bytes, s);
 }
 
 hipError_t launch_probe(const ProbeConfig& c, const ProbeArgs& a, int grid, size_t lds_bytes, hipStream_t s) {

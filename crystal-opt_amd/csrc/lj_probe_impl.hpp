@@ -572,14 +572,14 @@ __device__ __forceinline__ void load_tile(const int32_t* col, uint32_t base, uin
 // for the next query, reduce the per-workgroup statistics.  `part` is LDS
 // scratch of WG long longs.
 template <int WG>
-__device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* part) {
+__device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* part, uint32_t nblk) {
   constexpr int ROWS = WG / kStats;
   const uint32_t n = __hip_atomic_load(a.n_touched, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   {
     const int k = threadIdx.x % kStats, r = threadIdx.x / kStats;
     long long acc = 0;
     if (k != ST_ORDER)
-      for (int b = r; b < (int)gridDim.x; b += ROWS) acc += a.stats[(int64_t)b * kStats + k];
+      for (int b = r; b < (int)nblk; b += ROWS) acc += a.stats[(int64_t)b * kStats + k];
     part[r * kStats + k] = acc;
   }
   __syncthreads();
@@ -884,16 +884,10 @@ __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, co
   }
 }
 
+// The probe of one query by workgroup `bid` of the `nblk` workgroups that
+// serve it (the whole grid for a single query; a slice of it in a batch).
 template <int WG, int VEC, int K0>
-__global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
-  // Read the arguments in place from the kernarg segment: indexing the
-  // by-value copy with a (uniform) dimension index would copy it to scratch.
-#if defined(__HIP_DEVICE_COMPILE__)
-  (void)args;
-  const ProbeArgs& a = *(const ProbeArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
-#else
-  const ProbeArgs& a = args;
-#endif
+__device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bid, const uint32_t nblk) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   constexpr int NW = WG / 64;
   const int wave = uniform(threadIdx.x >> 6);  // known wave-uniform: keeps per-wave branches scalar
@@ -918,7 +912,7 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   off += (kMaxDims + 1) * 8;
   uint32_t* lds_next = reinterpret_cast<uint32_t*>(lds_raw + off);  // tile counter
 
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.out_hdr[kClockSlot + 1] = (long long)wall_clock64();
+  if (bid == 0 && threadIdx.x == 0) a.out_hdr[kClockSlot + 1] = (long long)wall_clock64();
 
   // ---- stage the LDS-resident filters, clear the group table
   for (int d = 0; d < a.n_dims; d++) {
@@ -964,8 +958,8 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   w.ld2 = a.load_order[2];
   w.ld3 = a.load_order[3];
 
-  const uint32_t total_waves = gridDim.x * NW;
-  const uint32_t gw = blockIdx.x * NW + wave;
+  const uint32_t total_waves = nblk * NW;
+  const uint32_t gw = bid * NW + wave;
   const int64_t n64 = a.n;
   int oob = 0;
   bool eager = false;  // prefetch the next column with the first one
@@ -1023,7 +1017,7 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
       TileSched<NW> sch;
       sch.next = lds_next;
       sch.stride = total_waves;
-      sch.wg_base = blockIdx.x * NW;
+      sch.wg_base = bid * NW;
       if (eager) stream_loop<VEC, K0, true, NW>(a, s0, lds_filter, gt, w, st, tc, sch, n_full, oob);
       else stream_loop<VEC, K0, false, NW>(a, s0, lds_filter, gt, w, st, tc, sch, n_full, oob);
     }
@@ -1043,7 +1037,7 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
         if (i != ST_ORDER && st.v[i]) atomicAdd((unsigned long long*)&lds_stats[i], (unsigned long long)st.v[i]);
       atomicAdd((unsigned long long*)&lds_stats[ST_ROWS_IN + a.n_dims], (unsigned long long)st.meas_rows);
       atomicAdd((unsigned long long*)&lds_stats[ST_LINES_IN + a.n_dims], (unsigned long long)st.meas_lines);
-      if (blockIdx.x == 0 && wave == 0)
+      if (bid == 0 && wave == 0)
         lds_stats[ST_ORDER] = (long long)(w.ld0 & 0xff) | ((long long)(w.ld1 & 0xff) << 8) |
                               ((long long)(w.ld2 & 0xff) << 16) | ((long long)(w.ld3 & 0xff) << 24);
     }
@@ -1057,7 +1051,7 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
       global_group_add(a, g, (long long)gt.sum[i], gt.cnt[i], gt.prod[i], pay, a.n_dims);
     }
   }
-  for (int i = threadIdx.x; i < kStats; i += WG) a.stats[(int64_t)blockIdx.x * kStats + i] = lds_stats[i];
+  for (int i = threadIdx.x; i < kStats; i += WG) a.stats[(int64_t)bid * kStats + i] = lds_stats[i];
 
   // ---- the last workgroup to finish collects the result ---------------------------
   // Each workgroup publishes its writes with ONE agent-scope release (the
@@ -1068,14 +1062,43 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   uint32_t* ticket = reinterpret_cast<uint32_t*>(lds_samp);
   if (threadIdx.x == 0) *ticket = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
-  if (*ticket == gridDim.x - 1) {
+  if (*ticket == nblk - 1) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    finalize_block<WG>(a, reinterpret_cast<long long*>(lds_raw));
+    finalize_block<WG>(a, reinterpret_cast<long long*>(lds_raw), nblk);
     if (threadIdx.x == 0) {
       *a.done = 0;
       a.out_hdr[kClockSlot + 2] = (long long)wall_clock64();
     }
   }
+}
+
+template <int WG, int VEC, int K0>
+__global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
+  // Read the arguments in place from the kernarg segment: indexing the
+  // by-value copy with a (uniform) dimension index would copy it to scratch.
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)args;
+  const ProbeArgs& a = *(const ProbeArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
+#else
+  const ProbeArgs& a = args;
+#endif
+  probe_body<WG, VEC, K0>(a, blockIdx.x, gridDim.x);
+}
+
+// A batch of queries in one launch: query q is served by workgroups
+// [wg_begin[q], wg_begin[q+1]) (wg_begin[n_q] == gridDim.x); each workgroup
+// finds its query by binary search and runs the same probe on the query's
+// arguments, read from device memory.
+template <int WG, int VEC, int K0>
+__global__ __launch_bounds__(WG) void probe_multi_kernel(const ProbeArgs* qs, const uint32_t* wg_begin, int32_t n_q) {
+  int lo = 0, hi = n_q - 1;
+  while (lo < hi) {  // last q with wg_begin[q] <= blockIdx.x (uniform)
+    const int mid = (lo + hi + 1) >> 1;
+    if (wg_begin[mid] <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const uint32_t b0 = wg_begin[lo];
+  probe_body<WG, VEC, K0>(qs[lo], blockIdx.x - b0, wg_begin[lo + 1] - b0);
 }
 
 // Host-side table of instantiated kernels, filled by the instantiation units.

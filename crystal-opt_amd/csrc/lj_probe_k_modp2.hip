@@ -2,7 +2,9 @@
 #include "lj_probe_impl.hpp"
 
 namespace lj {
-#define LJ_ENTRY(W, V, K) {W, V, K, reinterpret_cast<const void*>(&probe_kernel<W, V, K>)},
+#define LJ_ENTRY(W, V, K) \
+  {W, V, K, reinterpret_cast<const void*>(&probe_kernel<W, V, K>), \
+   reinterpret_cast<const void*>(&probe_multi_kernel<W, V, K>)},
 static const ProbeEntry k_entries[] = {
     LJ_PROBE_SHAPES(LJ_ENTRY, S0_MODP2_LDS)
     LJ_PROBE_SHAPES(LJ_ENTRY, S0_MODP2_GL)

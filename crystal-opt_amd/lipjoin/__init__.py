@@ -10,6 +10,7 @@ as raw device addresses.  There is no CPU fallback: every call runs the HIP
 kernels of liblipjoin.so, and a missing library raises LibraryMissing.
 """
 import ctypes as C
+import weakref
 
 from . import _abi
 from ._abi import (  # noqa: F401  (re-exported constants)
@@ -135,6 +136,7 @@ class Context:
                     stream = torch.cuda.current_stream(device).cuda_stream
             except ImportError:
                 stream = None
+        self._batches = weakref.WeakSet()
         st = L.lj_create(int(device), int(hbm_budget), stream, C.byref(h))
         if st != _abi.OK:
             raise LJError(st, L.lj_last_error(None).decode())
@@ -143,6 +145,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for b in list(getattr(self, "_batches", ())):
+                b.close()   # a batch holds context memory: it goes first
             lib().lj_destroy(self.h)
             self.h = None
 
@@ -209,10 +213,74 @@ class Context:
             "time_probe_ms": r.time_probe_ms, "stats": stats_dict(r.stats), "status": st,
         }
 
+    # -- batches (many queries, one build launch + one probe launch per kernel kind) --------
+    def batch(self, descs, use_graph=False, groups_cap=64, probe_wg=0):
+        return Batch(self, descs, use_graph, groups_cap, probe_wg)
+
+    # -- Bloom autotuner ------------------------------------------------------------------
+    def tune_bloom(self, n_selected, n_dim_rows, fact_pass):
+        ch = _abi.BloomChoice()
+        self._check(lib().lj_tune_bloom(self.h, int(n_selected), int(n_dim_rows), float(fact_pass), C.byref(ch)))
+        return {k: getattr(ch, k) for k, _ in _abi.BloomChoice._fields_}
+
+    def tune_calibrate(self):
+        self._check(lib().lj_tune_calibrate(self.h))
+
     # -- synthetic columns -------------------------------------------------------------
     def gen_device(self, out, kind, seed, stream, lo, hi, first=0, n=None):
         n = out.numel() if n is None else n
         self._check(lib().lj_gen_device(self.h, kind, seed, stream, lo, hi, first, n, _ptr(out)))
+
+
+def _star_result_dict(r, arr, n_dims, cap):
+    n = min(r.n_groups, cap)
+    groups = [(arr[i].g, tuple(arr[i].payload[:n_dims]), arr[i].sum, arr[i].count, arr[i].sum_i32)
+              for i in range(n)]
+    return {
+        "groups": groups, "n_groups": r.n_groups, "n_rows": r.n_rows, "sum_total": r.sum_total,
+        "time_query_ms": r.time_query_ms, "time_build_ms": r.time_build_ms,
+        "time_probe_ms": r.time_probe_ms, "stats": stats_dict(r.stats),
+    }
+
+
+class Batch:
+    """lj_batch: the queries run as one build launch and one probe launch per
+    stage-0 kernel kind; with use_graph the launches are recorded once as a HIP
+    graph and replayed."""
+
+    def __init__(self, ctx, descs, use_graph=False, groups_cap=64, probe_wg=0):
+        self.ctx = ctx
+        self.n = len(descs)
+        self._descs = (_abi.StarDesc * self.n)(*descs)
+        self._n_dims = [d.n_dims for d in descs]
+        o = _abi.BatchOpts()
+        o.use_graph, o.probe_wg, o.groups_cap = int(bool(use_graph)), int(probe_wg), int(groups_cap)
+        self.cap = int(groups_cap)
+        h = C.c_void_p()
+        ctx._check(lib().lj_batch_create(ctx.h, self._descs, self.n, C.byref(o), C.byref(h)))
+        self.h = h
+        ctx._batches.add(self)
+        self._res = (_abi.StarResult * self.n)()
+        self._arrs = [(_abi.Group * self.cap)() for _ in range(self.n)]
+        for i in range(self.n):
+            self._res[i].groups = C.cast(self._arrs[i], C.POINTER(_abi.Group))
+            self._res[i].groups_cap = self.cap
+
+    def run(self, allow_domain=False):
+        st = lib().lj_batch_run(self.h, self._res)
+        self.ctx._check(st, allow=(_abi.EDOMAIN,) if allow_domain else ())
+        return [_star_result_dict(self._res[i], self._arrs[i], self._n_dims[i], self.cap) for i in range(self.n)]
+
+    def close(self):
+        if self.h:
+            lib().lj_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gen_host(kind, seed, stream, lo, hi, n, first=0):

@@ -107,7 +107,18 @@ class StarResult(C.Structure):
     ]
 
 
-STRUCTS = [DimDesc, DimInfo, ProbeStats, JoinDesc, JoinResult, StarDesc, Group, StarResult]
+class BatchOpts(C.Structure):
+    _fields_ = [("use_graph", C.c_int32), ("probe_wg", C.c_int32), ("groups_cap", C.c_int64),
+                ("reserved", C.c_int64 * 2)]
+
+
+class BloomChoice(C.Structure):
+    _fields_ = [("bloom_hash", C.c_int32), ("bloom_layout", C.c_int32), ("bloom_k", C.c_int32),
+                ("in_lds", C.c_int32), ("bloom_bits", C.c_int64), ("fpr", C.c_double), ("cost", C.c_double)]
+
+
+STRUCTS = [DimDesc, DimInfo, ProbeStats, JoinDesc, JoinResult, StarDesc, Group, StarResult, BatchOpts,
+           BloomChoice]
 
 # name -> (restype, argtypes); every entry point of include/lipjoin.h
 SIGNATURES = {
@@ -125,6 +136,12 @@ SIGNATURES = {
     "lj_dim_release": (None, [C.c_void_p, C.c_void_p]),
     "lj_hash_join": (C.c_int, [C.c_void_p, C.POINTER(JoinDesc), C.POINTER(JoinResult)]),
     "lj_star_join": (C.c_int, [C.c_void_p, C.POINTER(StarDesc), C.POINTER(StarResult)]),
+    "lj_batch_create": (C.c_int, [C.c_void_p, C.POINTER(StarDesc), C.c_int32, C.POINTER(BatchOpts),
+                                  C.POINTER(C.c_void_p)]),
+    "lj_batch_run": (C.c_int, [C.c_void_p, C.POINTER(StarResult)]),
+    "lj_batch_destroy": (None, [C.c_void_p]),
+    "lj_tune_bloom": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.POINTER(BloomChoice)]),
+    "lj_tune_calibrate": (C.c_int, [C.c_void_p]),
     "lj_gen_host": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
                               C.c_int64, C.c_void_p]),
     "lj_gen_device": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_uint64, C.c_int64, C.c_int64,

@@ -180,4 +180,61 @@ def single_join_pred(threshold):
     return ((COMB_SET, CMP_LT, int(threshold)),)
 
 
+# ------------------------------------------------------------------ batches --
+# Config 4: many independent single joins shaped like the Bloom join: fact
+# rows floor(2^(lo + span*u)), u ~ U[0,1); a dimension of n_dim dense keys per
+# instance (attr ~ U[0,9999]); selectivity s ~ U[s_lo, s_hi] percent with the
+# predicate attr < round(100*s); the Bloom filter chosen by the autotuner.
+def batch_specs(seed, n_inst=64, log_lo=20, log_span=7, n_dim=1 << 22, s_lo=0.5, s_hi=20.0):
+    rng = np.random.default_rng(seed)
+    u = rng.random(n_inst)
+    sel = rng.uniform(s_lo, s_hi, n_inst)
+    out = []
+    for i in range(n_inst):
+        out.append({"n_fact": int(np.floor(2.0 ** (log_lo + log_span * u[i]))), "n_dim": int(n_dim),
+                    "threshold": int(round(100 * sel[i])), "sel_pct": float(sel[i]),
+                    "seed": int(seed) * 1000 + i})
+    return out
+
+
+def batch_instance_host(spec):
+    """Host columns of one instance (the same bits as batch_instance_device)."""
+    n_dim, n_fact, seed = spec["n_dim"], spec["n_fact"], spec["seed"]
+    keys = np.arange(n_dim, dtype=np.int32)
+    sp = single_join_specs(n_dim, 9999)
+    attr = gen_host(sp["dim_attr"][0], seed, *sp["dim_attr"][1:], n_dim)
+    fk = gen_host(sp["fact_fk"][0], seed, *sp["fact_fk"][1:], n_fact)
+    meas = gen_host(sp["fact_measure"][0], seed, *sp["fact_measure"][1:], n_fact)
+    return keys, attr, fk, meas
+
+
+def batch_instance_device(ctx, spec, torch_device="cuda"):
+    import torch
+    n_dim, n_fact, seed = spec["n_dim"], spec["n_fact"], spec["seed"]
+    sp = single_join_specs(n_dim, 9999)
+    cols = {}
+    cols["keys"] = torch.empty(n_dim, dtype=torch.int32, device=torch_device)
+    ctx.gen_device(cols["keys"], GEN_SEQUENCE, seed, 0, 0, 0)
+    for name, key, n in (("attr", "dim_attr", n_dim), ("fk", "fact_fk", n_fact), ("meas", "fact_measure", n_fact)):
+        cols[name] = torch.empty(max(n, 1), dtype=torch.int32, device=torch_device)
+        ctx.gen_device(cols[name], sp[key][0], seed, *sp[key][1:], n=n)
+    return cols
+
+
+def batch_desc(cols, spec, method="lip", bloom=None):
+    """Star descriptor (one dimension, one group) of a batch instance."""
+    d = dim_desc(cols["keys"], spec["n_dim"], spec["n_dim"], attr=cols["attr"],
+                 predicate=single_join_pred(spec["threshold"]), bloom=bloom)
+    return star_desc([cols["fk"]], cols["meas"], spec["n_fact"], [d], (0,), method=method)
+
+
+def tuned_bloom(ctx, spec):
+    """The autotuner's Bloom filter for an instance, as a dim_desc bloom tuple."""
+    n_sel = int(round(spec["n_dim"] * spec["threshold"] / 10000.0))
+    ch = ctx.tune_bloom(n_sel, spec["n_dim"], spec["threshold"] / 10000.0)
+    if ch["bloom_hash"] == BLOOM_MOD:
+        return ("mod", ch["bloom_bits"]), ch
+    return ("mult", ch["bloom_bits"], ch["bloom_k"], ch["bloom_layout"]), ch
+
+
 __all__ = [n for n in dir() if not n.startswith("_")] + ["BLOOM_MOD"]

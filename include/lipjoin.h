@@ -232,6 +232,57 @@ typedef struct {
 
 lj_status lj_star_join(lj_ctx* ctx, const lj_star_desc* desc, lj_star_result* out);
 
+/* ---- batches of queries ---------------------------------------------------------
+ * n independent star joins (any mix of sizes, predicates and Bloom filters)
+ * run as ONE build launch over all their dimensions and ONE probe launch per
+ * stage-0 kernel kind (one in total when the filters share a kind), the
+ * workgroups of the grid partitioned over the queries in proportion to their
+ * fact rows.  With use_graph, the launches (table clears, builds, probes and
+ * the result copy) are recorded once as a HIP graph and replayed by every
+ * later lj_batch_run.  Results are identical to n calls of lj_star_join.
+ * No reference counterpart: the reference runs one query per launch
+ * sequence; this is the batched mode of the engine. */
+typedef struct lj_batch lj_batch;
+
+typedef struct {
+  int32_t use_graph;       /* record the launches as a HIP graph on the first run */
+  int32_t probe_wg;        /* workgroup size 256/512/1024; 0: the context's choice */
+  int64_t groups_cap;      /* result groups kept per query on the device (>= 1; 0: 64) */
+  int64_t reserved[2];
+} lj_batch_opts;
+
+/* descs[0..n): every dimension built inside the batch (dims[d]) or resident.
+ * Device columns must stay valid until lj_batch_destroy. */
+lj_status lj_batch_create(lj_ctx* ctx, const lj_star_desc* descs, int32_t n, const lj_batch_opts* opts,
+                          lj_batch** out);
+/* Runs the batch; results[i] is filled like lj_star_join's (groups ascending g,
+ * at most groups_cap of them); time_query_ms is the whole batch's time. */
+lj_status lj_batch_run(lj_batch* b, lj_star_result* results);
+void lj_batch_destroy(lj_batch* b);
+
+/* ---- Bloom filter autotuner ------------------------------------------------------
+ * Picks the Bloom filter (hash, layout, k, bits) of a LIP join on one
+ * dimension: n_selected keys pass the dimension predicate, the fact side has
+ * n_fact rows of which a fraction fact_pass joins.  The cost model per fact
+ * row is scan(bits, k, in LDS or not) + P(pass) * survivor, with
+ * P(pass) = fact_pass + (1 - fact_pass) * false-positive rate(bits, k, layout);
+ * the constants are the context's (measured defaults, or lj_tune_calibrate). */
+typedef struct {
+  int32_t bloom_hash;      /* lj_bloom_hash */
+  int32_t bloom_layout;    /* lj_bloom_layout */
+  int32_t bloom_k;
+  int32_t in_lds;          /* 1 if the filter fits the probe's LDS */
+  int64_t bloom_bits;
+  double fpr;              /* predicted false-positive rate */
+  double cost;             /* predicted cost per fact row (scan of an LDS filter = 1) */
+} lj_bloom_choice;
+
+lj_status lj_tune_bloom(lj_ctx* ctx, int64_t n_selected, int64_t n_dim_rows, double fact_pass,
+                        lj_bloom_choice* out);
+/* Measures the model's constants on this device (a few probes of synthetic
+ * data, about a second) and keeps them in the context. */
+lj_status lj_tune_calibrate(lj_ctx* ctx);
+
 /* ---- seeded synthetic columns ------------------------------------------------ */
 /* Counter-based generator: value j = first + i of stream `stream` depends only
  * on (seed, stream, j), so host and device produce identical columns. */
