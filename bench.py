@@ -49,6 +49,62 @@ def measured_traffic():
         return None
 
 
+def bench_batch(ctx, n_inst, reps):
+    """Config 4: n_inst single joins (fact rows 2^20..2^27 log-uniform, 2^22-key
+    dimension each, selectivity 0.5-20 %, Bloom filter from the autotuner) as one
+    batch (graph replay and eager) against n_inst separate lj_star_join calls."""
+    import torch
+    from lipjoin import workloads as W
+    ctx.tune_calibrate()
+    specs = W.batch_specs(W.seed_of(4), n_inst=n_inst)
+    cols, descs, choices = [], [], []
+    for sp in specs:
+        c = W.batch_instance_device(ctx, sp)
+        bloom, ch = W.tuned_bloom(ctx, sp)
+        cols.append(c)
+        choices.append(ch)
+        descs.append(W.batch_desc(c, sp, method="lip", bloom=bloom))
+    torch.cuda.synchronize()
+    rows = sum(sp["n_fact"] for sp in specs)
+    sep = [ctx.star_join(d) for d in descs]          # warm-up and reference results
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for d in descs:
+            ctx.star_join(d)
+        t.append(time.perf_counter() - t0)
+    sep_s = statistics.median(t)
+    out = {"instances": n_inst, "fact_rows_total": rows, "separate_ms": sep_s * 1e3}
+    for mode, graph in (("eager", False), ("graph", True)):
+        b = ctx.batch(descs, use_graph=graph)
+        res = b.run()
+        exact = all((r["sum_total"], r["n_rows"]) == (s["sum_total"], s["n_rows"]) for r, s in zip(res, sep))
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res = b.run()
+            t.append(time.perf_counter() - t0)
+        exact = exact and all((r["sum_total"], r["n_rows"]) == (s["sum_total"], s["n_rows"])
+                              for r, s in zip(res, sep))
+        b.close()
+        ms = statistics.median(t) * 1e3
+        out[f"batch_{mode}_ms"] = ms
+        out[f"batch_{mode}_rows_per_s"] = rows / (ms * 1e-3)
+        out[f"speedup_{mode}_vs_separate"] = sep_s * 1e3 / ms
+        out[f"bit_exact_{mode}"] = exact
+    kinds = {}
+    for ch in choices:
+        k = f"{'mod' if ch['bloom_hash'] == 1 else 'mult'}-k{ch['bloom_k']}-{'blk' if ch['bloom_layout'] else 'std'}" \
+            f"-2^{int(ch['bloom_bits']).bit_length() - 1}{'-lds' if ch['in_lds'] else ''}"
+        kinds[k] = kinds.get(k, 0) + 1
+    out["bloom_choices"] = kinds
+    out["separate_rows_per_s"] = rows / sep_s
+    log(f"batch: {n_inst} joins, {rows / 1e9:.2f} G rows: separate {sep_s * 1e3:.2f} ms, "
+        f"eager {out['batch_eager_ms']:.2f} ms, graph {out['batch_graph_ms']:.2f} ms")
+    del cols
+    return out
+
+
 def cpu_baseline(n_fact, seed, threads):
     """Oracle (C restatement of the reference semantics) on the host cores:
     the LIP plan in join order S->C->D, whole 120M-row input, median of 5
@@ -83,6 +139,8 @@ def main():
     ap.add_argument("--plan-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plans", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=64,
+                    help="config-4 batch size (0 skips the batched measurement)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -198,6 +256,11 @@ def main():
         "lip_vs_naive_speedup": speedup,
         "plans": plans,
     }
+    if args.batch_instances > 0:
+        # free the 120M-row columns first
+        del cols, descs
+        torch.cuda.empty_cache()
+        out["batched"] = bench_batch(ctx, args.batch_instances, reps=5)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")

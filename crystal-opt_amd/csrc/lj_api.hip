@@ -1122,20 +1122,21 @@ extern "C" lj_status lj_batch_create(lj_ctx* c, const lj_star_desc* descs, int32
     K.qs.push_back(i);
     bq.group = k;
   }
-  // workgroups: about four per resident slot of the kind, in proportion to the
-  // fact rows, at least one per query and no more than it has tiles for
+  // workgroups: each query gets as many as one query alone would (the
+  // resident slots of the kind, fewer if it has fewer tiles), in query order.
+  // The dispatcher starts workgroups in order, so the queries run nearly one
+  // after another -- each query's filters and hash table keep the caches to
+  // themselves -- with the next query filling the previous one's tail and no
+  // launch gap between them.
   for (BatchKind& K : b->kinds) {
     lj_status s = occupancy_of(c, K.cfg, K.lds, &K.per_cu);
     if (s != LJ_OK) return bail(s);
     const int64_t slots = (int64_t)c->n_cu * K.per_cu;
-    int64_t rows = 0;
-    for (int i : K.qs) rows += b->q[(size_t)i].desc.n_fact;
     const int64_t tile_rows = (int64_t)K.cfg.vec * kSegRows * (K.cfg.wg / 64);
     for (int i : K.qs) {
       BatchQuery& bq = b->q[(size_t)i];
       const int64_t need = std::max<int64_t>(1, (bq.desc.n_fact + tile_rows - 1) / tile_rows);
-      int64_t g = rows > 0 ? (int64_t)((double)bq.desc.n_fact / (double)rows * (double)(4 * slots) + 0.5) : 1;
-      bq.grid = std::min(std::max<int64_t>(g, 1), need);
+      bq.grid = std::min(slots, need);
       K.grid += bq.grid;
     }
     if (K.grid >= (1ll << 31)) return bail(fail(c, LJ_EUNSUPPORTED, "batch grid too large"));

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs args) {
 }
 
 // The dimensions of a batch (descriptors in device memory).
-__global__ __launch_bounds__(kBuildThreads) void build_multi_kernel(const BuildDim* dims, int32_t n) {
+__global__ __launch_bounds__(kBuildThreads) void build_multi_kernel(const BuildDim* __restrict__ dims, int32_t n) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {  // last dimension with block_begin <= blockIdx.x (uniform)
     const int mid = (lo + hi + 1) >> 1;

@@ -1090,7 +1090,10 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
 // finds its query by binary search and runs the same probe on the query's
 // arguments, read from device memory.
 template <int WG, int VEC, int K0>
-__global__ __launch_bounds__(WG) void probe_multi_kernel(const ProbeArgs* qs, const uint32_t* wg_begin, int32_t n_q) {
+__global__ __launch_bounds__(WG) void probe_multi_kernel(const ProbeArgs* __restrict__ qs,
+                                                         const uint32_t* __restrict__ wg_begin, int32_t n_q) {
+  // __restrict__: the arguments do not alias what the probe writes, so their
+  // fields load once instead of after every store
   int lo = 0, hi = n_q - 1;
   while (lo < hi) {  // last q with wg_begin[q] <= blockIdx.x (uniform)
     const int mid = (lo + hi + 1) >> 1;
