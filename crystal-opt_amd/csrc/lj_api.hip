@@ -13,6 +13,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -85,6 +89,7 @@ struct lj_ctx {
   double tune_surv = 40.0;       // a row that passes the filter (queue, gathers, hash probe)
   int tune_calibrated = 0;
   std::map<std::pair<int, size_t>, int> occupancy;  // (config key, LDS bytes) -> workgroups per CU
+  std::multimap<size_t, void*> pinned_free;         // pinned host blocks for reuse (batches)
 };
 
 static const int kHostGroups = 4096;   // groups the pinned host buffer holds
@@ -216,6 +221,23 @@ static lj_status dev_alloc_zero_t(lj_ctx* c, int64_t count, T** out) {
   return s;
 }
 
+// Pinned host memory (power-of-two blocks, kept by the context for reuse).
+static lj_status host_alloc(lj_ctx* c, size_t bytes, void** out) {
+  const size_t sz = round_block(bytes ? bytes : 1);
+  auto it = c->pinned_free.find(sz);
+  if (it != c->pinned_free.end()) {
+    *out = it->second;
+    c->pinned_free.erase(it);
+    return LJ_OK;
+  }
+  HIP_TRY(c, hipHostMalloc(out, sz, hipHostMallocDefault));
+  return LJ_OK;
+}
+
+static void host_release(lj_ctx* c, void* p, size_t bytes) {
+  if (p) c->pinned_free.emplace(round_block(bytes ? bytes : 1), p);
+}
+
 template <typename T>
 static lj_status dev_alloc_t(lj_ctx* c, int64_t count, T** out) {
   void* p = nullptr;
@@ -239,6 +261,8 @@ extern "C" int64_t lj_abi_struct_size(int32_t which) {
     case 7: return sizeof(lj_star_result);
     case 8: return sizeof(lj_batch_opts);
     case 9: return sizeof(lj_bloom_choice);
+    case 10: return sizeof(lj_serve_cfg);
+    case 11: return sizeof(lj_serve_stats);
   }
   return -1;
 }
@@ -319,6 +343,7 @@ extern "C" void lj_destroy(lj_ctx* c) {
   for (auto& ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->host_buf) (void)hipHostFree(c->host_buf);
+  for (auto& kv : c->pinned_free) (void)hipHostFree(kv.second);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1048,7 +1073,10 @@ struct lj_batch {
   char* out_host = nullptr;
   size_t out_bytes = 0;
   void* args_buf = nullptr;       // ProbeArgs[] and wg_begin[] of every kind, BuildDim[]
+  void* args_host = nullptr;      // pinned staging of args_buf
+  size_t args_bytes = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1058,7 +1086,9 @@ struct lj_batch {
 static void batch_free(lj_batch* b) {
   if (!b) return;
   lj_ctx* c = b->c;
-  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->stream && b->own_stream) (void)hipStreamSynchronize(b->stream);
+  else if (b->stream && b->e1 && b->runs > 0) (void)hipEventSynchronize(b->e1);  // a shared stream: only this batch
+  else if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (auto& bq : b->q) dims_free(c, bq.p.owned, bq.p.n_owned);
@@ -1070,21 +1100,26 @@ static void batch_free(lj_batch* b) {
   dev_release(c, b->plain_buf);
   dev_release(c, b->out_dev);
   dev_release(c, b->args_buf);
-  if (b->out_host) (void)hipHostFree(b->out_host);
+  host_release(c, b->out_host, b->out_bytes);
+  host_release(c, b->args_host, b->args_bytes);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
-  if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->stream && b->own_stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
-extern "C" lj_status lj_batch_create(lj_ctx* c, const lj_star_desc* descs, int32_t n, const lj_batch_opts* opts,
-                                     lj_batch** out) {
-  if (!c || !out || (n > 0 && !descs)) return fail(c, LJ_EINVAL, "NULL argument");
+// Plans and allocates a batch and queues the upload of its arguments on
+// `stream` (the batch's own stream when NULL); nothing waits for the device.
+static lj_status batch_build(lj_ctx* c, const lj_star_desc* descs, int32_t n, const lj_batch_opts* opts,
+                             hipStream_t stream, lj_batch** out) {
   *out = nullptr;
   if (n < 1 || n > (1 << 16)) return fail(c, LJ_EINVAL, "batch of %d queries (1..65536)", n);
-  HIP_TRY(c, hipSetDevice(c->device));
   lj_batch* b = new lj_batch();
   b->c = c;
+  if (stream) {
+    b->stream = stream;
+    b->own_stream = false;
+  }
   if (opts) b->opts = *opts;
   if (b->opts.groups_cap <= 0) b->opts.groups_cap = 64;
   const int wg = b->opts.probe_wg;
@@ -1165,7 +1200,9 @@ extern "C" lj_status lj_batch_create(lj_ctx* c, const lj_star_desc* descs, int32
   if ((s = dev_alloc(c, out_bytes, &op)) != LJ_OK) return bail(s);
   b->out_dev = static_cast<char*>(op);
   b->out_bytes = out_bytes;
-  HIP_TRY(c, hipHostMalloc((void**)&b->out_host, out_bytes, hipHostMallocDefault));
+  void* oh = nullptr;
+  if ((s = host_alloc(c, out_bytes, &oh)) != LJ_OK) return bail(s);
+  b->out_host = static_cast<char*>(oh);
   // the argument arrays of every kind and the build descriptors
   size_t args_bytes = 0;
   std::vector<size_t> qs_off(b->kinds.size()), wgb_off(b->kinds.size());
@@ -1188,14 +1225,17 @@ extern "C" lj_status lj_batch_create(lj_ctx* c, const lj_star_desc* descs, int32
   const size_t build_off = args_bytes;
   args_bytes += align256(sizeof(BuildDim) * std::max<size_t>(bds.size(), 1));
   if ((s = dev_alloc(c, args_bytes, &b->args_buf)) != LJ_OK) return bail(s);
+  b->args_bytes = args_bytes;
+  if ((s = host_alloc(c, args_bytes, &b->args_host)) != LJ_OK) return bail(s);
   char* args = static_cast<char*>(b->args_buf);
-  std::vector<char> host_args(args_bytes, 0);
+  char* host_args = static_cast<char*>(b->args_host);
+  memset(host_args, 0, args_bytes);
   for (size_t k = 0; k < b->kinds.size(); k++) {
     BatchKind& K = b->kinds[k];
     K.d_qs = reinterpret_cast<ProbeArgs*>(args + qs_off[k]);
     K.d_wgb = reinterpret_cast<uint32_t*>(args + wgb_off[k]);
-    ProbeArgs* hq = reinterpret_cast<ProbeArgs*>(host_args.data() + qs_off[k]);
-    uint32_t* hw = reinterpret_cast<uint32_t*>(host_args.data() + wgb_off[k]);
+    ProbeArgs* hq = reinterpret_cast<ProbeArgs*>(host_args + qs_off[k]);
+    uint32_t* hw = reinterpret_cast<uint32_t*>(host_args + wgb_off[k]);
     uint32_t begin = 0;
     for (size_t j = 0; j < K.qs.size(); j++) {
       BatchQuery& bq = b->q[(size_t)K.qs[j]];
@@ -1221,16 +1261,15 @@ extern "C" lj_status lj_batch_create(lj_ctx* c, const lj_star_desc* descs, int32
     }
     hw[K.qs.size()] = begin;
   }
-  if (!bds.empty()) memcpy(host_args.data() + build_off, bds.data(), sizeof(BuildDim) * bds.size());
+  if (!bds.empty()) memcpy(host_args + build_off, bds.data(), sizeof(BuildDim) * bds.size());
   b->d_build = reinterpret_cast<BuildDim*>(args + build_off);
-  HIP_TRY(c, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  HIP_TRY(c, hipEventCreate(&b->e0));
-  HIP_TRY(c, hipEventCreate(&b->e1));
+  if (!b->stream) HIP_TRY(c, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIP_TRY(c, hipEventCreateWithFlags(&b->e0, hipEventDefault));
+  HIP_TRY(c, hipEventCreateWithFlags(&b->e1, hipEventDefault));
   // after everything queued on the context's stream (zero-pool clears)
   HIP_TRY(c, hipEventRecord(b->e0, c->stream));
   HIP_TRY(c, hipStreamWaitEvent(b->stream, b->e0, 0));
-  HIP_TRY(c, hipMemcpyAsync(args, host_args.data(), args_bytes, hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(c, hipStreamSynchronize(b->stream));
+  HIP_TRY(c, hipMemcpyAsync(args, host_args, args_bytes, hipMemcpyHostToDevice, b->stream));
   // the tables to clear before a repeated run
   FillArgs f{};
   for (BatchQuery& bq : b->q)
@@ -1252,6 +1291,17 @@ extern "C" lj_status lj_batch_create(lj_ctx* c, const lj_star_desc* descs, int32
   return LJ_OK;
 }
 
+extern "C" lj_status lj_batch_create(lj_ctx* c, const lj_star_desc* descs, int32_t n, const lj_batch_opts* opts,
+                                     lj_batch** out) {
+  if (!c || !out || (n > 0 && !descs)) return fail(c, LJ_EINVAL, "NULL argument");
+  *out = nullptr;
+  HIP_TRY(c, hipSetDevice(c->device));
+  lj_status s = batch_build(c, descs, n, opts, nullptr, out);
+  if (s != LJ_OK) return s;
+  HIP_TRY(c, hipStreamSynchronize((*out)->stream));  // the arguments are on the device
+  return LJ_OK;
+}
+
 // The launches of one run (recorded once when use_graph).
 static lj_status batch_enqueue(lj_batch* b, bool clear) {
   lj_ctx* c = b->c;
@@ -1264,11 +1314,9 @@ static lj_status batch_enqueue(lj_batch* b, bool clear) {
   return LJ_OK;
 }
 
-extern "C" lj_status lj_batch_run(lj_batch* b, lj_star_result* results) {
-  if (!b) return fail(nullptr, LJ_EINVAL, "NULL batch");
+// Queues one run (graph replay or the launches); e0/e1 bracket it.
+static lj_status batch_launch(lj_batch* b) {
   lj_ctx* c = b->c;
-  if (!results) return fail(c, LJ_EINVAL, "NULL results");
-  HIP_TRY(c, hipSetDevice(c->device));
   lj_status s;
   if (b->opts.use_graph) {
     if (!b->exec) {
@@ -1292,8 +1340,14 @@ extern "C" lj_status lj_batch_run(lj_batch* b, lj_star_result* results) {
     if ((s = batch_enqueue(b, b->runs > 0)) != LJ_OK) return s;
     HIP_TRY(c, hipEventRecord(b->e1, b->stream));
   }
-  HIP_TRY(c, hipEventSynchronize(b->e1));
   b->runs++;
+  return LJ_OK;
+}
+
+// After e1 has completed: the results of the last run.
+static lj_status batch_collect(lj_batch* b, lj_star_result* results) {
+  lj_ctx* c = b->c;
+  lj_status s;
   float ms = 0.f;
   HIP_TRY(c, hipEventElapsedTime(&ms, b->e0, b->e1));
   lj_status first_err = LJ_OK;
@@ -1311,6 +1365,17 @@ extern "C" lj_status lj_batch_run(lj_batch* b, lj_star_result* results) {
     if (s != LJ_OK && first_err == LJ_OK) first_err = s;
   }
   return first_err;
+}
+
+extern "C" lj_status lj_batch_run(lj_batch* b, lj_star_result* results) {
+  if (!b) return fail(nullptr, LJ_EINVAL, "NULL batch");
+  lj_ctx* c = b->c;
+  if (!results) return fail(c, LJ_EINVAL, "NULL results");
+  HIP_TRY(c, hipSetDevice(c->device));
+  lj_status s = batch_launch(b);
+  if (s != LJ_OK) return s;
+  HIP_TRY(c, hipEventSynchronize(b->e1));
+  return batch_collect(b, results);
 }
 
 extern "C" void lj_batch_destroy(lj_batch* b) { batch_free(b); }
@@ -1453,5 +1518,272 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
   // t_pass scans an L2 filter (2^22 bits = 512 KB) and passes 10 % of the rows
   c->tune_surv = std::max(1.0, ((double)t_pass - t_gl) / 0.1 / t_lds);
   c->tune_calibrated = 1;
+  return LJ_OK;
+}
+
+// ------------------------------------------------------------------- serving --
+namespace {
+using Clock = std::chrono::steady_clock;
+
+struct ServeReq {
+  uint64_t id = 0;
+  lj_star_desc desc{};
+  Clock::time_point t_submit;
+  uint64_t bytes = 0;
+};
+
+struct ServeDone {
+  lj_status st = LJ_OK;
+  std::string err;
+  lj_star_result r{};
+  std::vector<lj_group> groups;
+  int32_t n_dims = 0;
+};
+
+struct InFlight {
+  lj_batch* b = nullptr;
+  std::vector<ServeReq> reqs;
+  uint64_t bytes = 0;
+  int slot = 0;
+};
+}  // namespace
+
+struct lj_server {
+  lj_ctx* c = nullptr;
+  lj_serve_cfg cfg{};
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv_work;   // the loop waits for requests
+  std::condition_variable cv_done;   // pollers wait for results
+  std::deque<ServeReq> pending;
+  std::unordered_map<uint64_t, ServeDone> done;
+  uint64_t next_id = 1;
+  bool stopping = false;
+  lj_serve_stats stats{};
+  std::vector<hipStream_t> streams;
+  std::vector<bool> slot_busy;
+};
+
+// Device memory a request holds while in flight (the batch's per-query
+// buffers): group accumulators and payloads, result region, statistics rows.
+static uint64_t serve_bytes(const lj_ctx* c, const lj_star_desc* q, int64_t groups_cap) {
+  const uint64_t ng = (uint64_t)q->n_groups;
+  const uint64_t tcap = std::min<uint64_t>(ng, (uint64_t)groups_cap);
+  const uint64_t slots = (uint64_t)c->n_cu * 2;
+  return align256(ng * 8) * 3 + 256 + align256(ng * kMaxDims * 4) + align256(tcap * 4) +
+         align256(slots * kStats * 8) + align256(kHdrBytes + sizeof(GroupOut) * tcap) + sizeof(ProbeArgs);
+}
+
+static void serve_finish(lj_server* S, const ServeReq& rq, lj_status st, const char* err, const lj_star_result* r,
+                         const lj_group* groups, Clock::time_point t_done) {
+  ServeDone d;
+  d.st = st;
+  if (err) d.err = err;
+  d.n_dims = rq.desc.n_dims;
+  if (r) {
+    d.r = *r;
+    const int64_t n = std::min<int64_t>(r->n_groups, r->groups_cap);
+    d.groups.assign(groups, groups + std::max<int64_t>(n, 0));
+  }
+  d.r.groups = nullptr;
+  d.r.time_query_ms = (float)std::chrono::duration<double, std::milli>(t_done - rq.t_submit).count();
+  S->done[rq.id] = std::move(d);
+}
+
+static void serve_loop(lj_server* S) {
+  lj_ctx* c = S->c;
+  (void)hipSetDevice(c->device);
+  std::vector<InFlight> inflight;
+  lj_batch_opts bopts{};
+  bopts.groups_cap = S->cfg.groups_cap;
+  for (;;) {
+    // 1. completions (the loop alone drives the context: no lock for HIP work)
+    for (size_t i = 0; i < inflight.size();) {
+      InFlight& f = inflight[i];
+      if (hipEventQuery(f.b->e1) != hipSuccess) {
+        i++;
+        continue;
+      }
+      const size_t n = f.reqs.size();
+      std::vector<lj_star_result> res(n);
+      std::vector<std::vector<lj_group>> gs(n, std::vector<lj_group>((size_t)S->cfg.groups_cap));
+      for (size_t j = 0; j < n; j++) {
+        res[j].groups = gs[j].data();
+        res[j].groups_cap = S->cfg.groups_cap;
+      }
+      const lj_status st = batch_collect(f.b, res.data());
+      const std::string err = st == LJ_OK ? std::string() : c->err;
+      batch_free(f.b);
+      const Clock::time_point t_done = Clock::now();
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (size_t j = 0; j < n; j++)
+          serve_finish(S, f.reqs[j], st, st == LJ_OK ? nullptr : err.c_str(), &res[j], gs[j].data(), t_done);
+        S->stats.completed += n;
+        S->stats.bytes_in_flight -= f.bytes;
+        S->slot_busy[(size_t)f.slot] = false;
+      }
+      S->cv_done.notify_all();
+      inflight.erase(inflight.begin() + (long)i);
+    }
+    // 2. admission: every waiting request the budget admits, into one launch per free slot
+    std::vector<ServeReq> take;
+    int slot = -1;
+    uint64_t take_bytes = 0;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      if (inflight.empty() && S->pending.empty()) {
+        if (S->stopping) break;
+        S->cv_work.wait(lk, [&] { return S->stopping || !S->pending.empty(); });
+        if (S->pending.empty()) break;
+      }
+      for (size_t k = 0; k < S->slot_busy.size(); k++)
+        if (!S->slot_busy[k]) slot = (int)k;
+      if (slot >= 0) {
+        const uint64_t budget = S->cfg.hbm_budget;
+        while (!S->pending.empty() && (int)take.size() < S->cfg.max_batch) {
+          ServeReq& rq = S->pending.front();
+          if (budget && rq.bytes > budget) {  // can never be admitted
+            serve_finish(S, rq, LJ_EBUDGET, "request needs more device memory than the serving budget", nullptr,
+                         nullptr, Clock::now());
+            S->stats.rejected++;
+            S->pending.pop_front();
+            S->cv_done.notify_all();
+            continue;
+          }
+          if (budget && S->stats.bytes_in_flight + take_bytes + rq.bytes > budget) break;
+          take_bytes += rq.bytes;
+          take.push_back(rq);
+          S->pending.pop_front();
+        }
+        if (!take.empty()) {
+          S->slot_busy[(size_t)slot] = true;
+          S->stats.bytes_in_flight += take_bytes;
+          S->stats.peak_bytes_in_flight = std::max(S->stats.peak_bytes_in_flight, S->stats.bytes_in_flight);
+          S->stats.launches++;
+          S->stats.max_batch_seen = std::max<uint64_t>(S->stats.max_batch_seen, take.size());
+        }
+      }
+    }
+    if (!take.empty()) {
+      std::vector<lj_star_desc> descs;
+      descs.reserve(take.size());
+      for (const ServeReq& rq : take) descs.push_back(rq.desc);
+      lj_batch* b = nullptr;
+      lj_status st = batch_build(c, descs.data(), (int32_t)descs.size(), &bopts, S->streams[(size_t)slot], &b);
+      if (st == LJ_OK) st = batch_launch(b);
+      if (st != LJ_OK) {
+        const std::string err = c->err;
+        if (b) batch_free(b);
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (const ServeReq& rq : take) serve_finish(S, rq, st, err.c_str(), nullptr, nullptr, Clock::now());
+        S->stats.rejected += take.size();
+        S->stats.bytes_in_flight -= take_bytes;
+        S->slot_busy[(size_t)slot] = false;
+        S->cv_done.notify_all();
+      } else {
+        InFlight f;
+        f.b = b;
+        f.reqs = std::move(take);
+        f.bytes = take_bytes;
+        f.slot = slot;
+        inflight.push_back(std::move(f));
+      }
+      continue;
+    }
+    if (!inflight.empty()) std::this_thread::sleep_for(std::chrono::microseconds(2));
+  }
+}
+
+extern "C" lj_status lj_serve_start(lj_ctx* c, const lj_serve_cfg* cfg, lj_server** out) {
+  if (!c || !out) return fail(c, LJ_EINVAL, "NULL argument");
+  *out = nullptr;
+  HIP_TRY(c, hipSetDevice(c->device));
+  lj_server* S = new lj_server();
+  S->c = c;
+  if (cfg) S->cfg = *cfg;
+  if (S->cfg.max_batch <= 0) S->cfg.max_batch = 64;
+  if (S->cfg.max_inflight <= 0) S->cfg.max_inflight = 2;
+  if (S->cfg.groups_cap <= 0) S->cfg.groups_cap = 64;
+  if (S->cfg.max_batch > 4096 || S->cfg.max_inflight > 8) {
+    delete S;
+    return fail(c, LJ_EINVAL, "max_batch <= 4096 and max_inflight <= 8");
+  }
+  for (int i = 0; i < S->cfg.max_inflight; i++) {
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      for (hipStream_t s2 : S->streams) (void)hipStreamDestroy(s2);
+      delete S;
+      return fail(c, LJ_EHIP, "hipStreamCreate failed");
+    }
+    S->streams.push_back(st);
+  }
+  S->slot_busy.assign((size_t)S->cfg.max_inflight, false);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  S->th = std::thread(serve_loop, S);
+  *out = S;
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_serve_submit(lj_server* S, const lj_star_desc* q, uint64_t* id) {
+  if (!S || !q || !id) return LJ_EINVAL;
+  for (int d = 0; d < q->n_dims && d < LJ_MAX_DIMS; d++)
+    if (!q->resident[d]) return LJ_EINVAL;  // serving runs over resident dimensions only
+  if (q->n_dims < 1 || q->n_dims > LJ_MAX_DIMS) return LJ_EINVAL;
+  ServeReq rq;
+  rq.desc = *q;
+  rq.t_submit = Clock::now();
+  rq.bytes = serve_bytes(S->c, q, S->cfg.groups_cap);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stopping) return LJ_ESTATE;
+    rq.id = S->next_id++;
+    *id = rq.id;
+    S->pending.push_back(rq);
+    S->stats.submitted++;
+  }
+  S->cv_work.notify_one();
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_serve_poll(lj_server* S, uint64_t id, lj_star_result* out, int32_t timeout_ms) {
+  if (!S || !out) return LJ_EINVAL;
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto ready = [&] { return S->done.count(id) != 0; };
+  if (!ready()) {
+    if (timeout_ms == 0) return LJ_ETIMEOUT;
+    if (timeout_ms < 0) S->cv_done.wait(lk, ready);
+    else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return LJ_ETIMEOUT;
+  }
+  ServeDone& d = S->done[id];
+  lj_group* g = out->groups;
+  const int64_t cap = out->groups_cap;
+  *out = d.r;
+  out->groups = g;
+  out->groups_cap = cap;
+  if (g)
+    for (int64_t i = 0; i < std::min<int64_t>(cap, (int64_t)d.groups.size()); i++) g[i] = d.groups[(size_t)i];
+  const lj_status st = d.st;
+  S->done.erase(id);
+  return st;
+}
+
+extern "C" lj_status lj_serve_get_stats(lj_server* S, lj_serve_stats* out) {
+  if (!S || !out) return LJ_EINVAL;
+  std::lock_guard<std::mutex> lk(S->mu);
+  *out = S->stats;
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_serve_stop(lj_server* S) {
+  if (!S) return LJ_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stopping = true;
+  }
+  S->cv_work.notify_all();
+  if (S->th.joinable()) S->th.join();
+  for (hipStream_t st : S->streams) (void)hipStreamDestroy(st);
+  delete S;
   return LJ_OK;
 }

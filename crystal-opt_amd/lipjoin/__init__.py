@@ -217,6 +217,10 @@ class Context:
     def batch(self, descs, use_graph=False, groups_cap=64, probe_wg=0):
         return Batch(self, descs, use_graph, groups_cap, probe_wg)
 
+    # -- serving (request loop over resident dimensions) ----------------------------------
+    def serve(self, hbm_budget=0, max_batch=64, max_inflight=2, groups_cap=64):
+        return Server(self, hbm_budget, max_batch, max_inflight, groups_cap)
+
     # -- Bloom autotuner ------------------------------------------------------------------
     def tune_bloom(self, n_selected, n_dim_rows, fact_pass):
         ch = _abi.BloomChoice()
@@ -275,6 +279,62 @@ class Batch:
         if self.h:
             lib().lj_batch_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """lj_server: submit star joins over resident dimensions from any thread,
+    poll for results.  The context must not be used otherwise until stop()."""
+
+    def __init__(self, ctx, hbm_budget=0, max_batch=64, max_inflight=2, groups_cap=64):
+        self.ctx = ctx
+        cfg = _abi.ServeCfg()
+        cfg.hbm_budget, cfg.max_batch, cfg.max_inflight, cfg.groups_cap = (int(hbm_budget), int(max_batch),
+                                                                          int(max_inflight), int(groups_cap))
+        self.cap = int(groups_cap)
+        h = C.c_void_p()
+        ctx._check(lib().lj_serve_start(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._n_dims = {}
+        ctx._batches.add(self)
+
+    def submit(self, desc):
+        i = C.c_uint64()
+        st = lib().lj_serve_submit(self.h, C.byref(desc), C.byref(i))
+        if st != _abi.OK:
+            raise LJError(st, "lj_serve_submit")
+        self._n_dims[i.value] = desc.n_dims
+        return i.value
+
+    def poll(self, rid, timeout_ms=-1):
+        """The result dict, or None on timeout (the request stays queued)."""
+        arr = (_abi.Group * self.cap)()
+        r = _abi.StarResult()
+        r.groups, r.groups_cap = C.cast(arr, C.POINTER(_abi.Group)), self.cap
+        st = lib().lj_serve_poll(self.h, int(rid), C.byref(r), int(timeout_ms))
+        if st == _abi.ETIMEOUT:
+            return None
+        nd = self._n_dims.pop(rid, 1)
+        out = _star_result_dict(r, arr, nd, self.cap)
+        out["status"] = st
+        return out
+
+    def stats(self):
+        s = _abi.ServeStats()
+        lib().lj_serve_get_stats(self.h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in _abi.ServeStats._fields_ if k != "reserved"}
+
+    def close(self):
+        if self.h:
+            lib().lj_serve_stop(self.h)
+            self.h = None
+
+    stop = close
 
     def __del__(self):
         try:

@@ -117,8 +117,19 @@ class BloomChoice(C.Structure):
                 ("in_lds", C.c_int32), ("bloom_bits", C.c_int64), ("fpr", C.c_double), ("cost", C.c_double)]
 
 
+class ServeCfg(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
+                ("groups_cap", C.c_int64), ("reserved", C.c_int64 * 2)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("launches", C.c_uint64), ("bytes_in_flight", C.c_uint64), ("peak_bytes_in_flight", C.c_uint64),
+                ("max_batch_seen", C.c_uint64), ("reserved", C.c_uint64)]
+
+
 STRUCTS = [DimDesc, DimInfo, ProbeStats, JoinDesc, JoinResult, StarDesc, Group, StarResult, BatchOpts,
-           BloomChoice]
+           BloomChoice, ServeCfg, ServeStats]
 
 # name -> (restype, argtypes); every entry point of include/lipjoin.h
 SIGNATURES = {
@@ -142,6 +153,11 @@ SIGNATURES = {
     "lj_batch_destroy": (None, [C.c_void_p]),
     "lj_tune_bloom": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.POINTER(BloomChoice)]),
     "lj_tune_calibrate": (C.c_int, [C.c_void_p]),
+    "lj_serve_start": (C.c_int, [C.c_void_p, C.POINTER(ServeCfg), C.POINTER(C.c_void_p)]),
+    "lj_serve_submit": (C.c_int, [C.c_void_p, C.POINTER(StarDesc), C.POINTER(C.c_uint64)]),
+    "lj_serve_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(StarResult), C.c_int32]),
+    "lj_serve_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServeStats)]),
+    "lj_serve_stop": (C.c_int, [C.c_void_p]),
     "lj_gen_host": (C.c_int, [C.c_int32, C.c_uint64, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
                               C.c_int64, C.c_void_p]),
     "lj_gen_device": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_uint64, C.c_int64, C.c_int64,

@@ -237,4 +237,47 @@ def tuned_bloom(ctx, spec):
     return ("mult", ch["bloom_bits"], ch["bloom_k"], ch["bloom_layout"]), ch
 
 
+# ------------------------------------------------------------------ serving --
+# Config 5: requests over resident dimensions.  A seeded pool of n_templates
+# predicate templates (attr < round(100*s), s ~ U[0.5, 20] %) over one
+# dimension of n_dim dense keys, each built once (PHT_2 + the autotuner's
+# Bloom filter); a resident fact pool of n_pool rows; a request is a template
+# and a slice [offset, offset + rows) of the pool, rows = floor(2^(lo + span*u)).
+def serve_templates(ctx, seed, n_dim=1 << 22, n_templates=8, torch_device="cuda"):
+    spec = {"n_dim": n_dim, "n_fact": 0, "seed": int(seed), "threshold": 0}
+    cols = batch_instance_device(ctx, spec, torch_device)
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(n_templates):
+        thr = int(round(100 * rng.uniform(0.5, 20.0)))
+        sp = dict(spec, threshold=thr)
+        bloom, _ = tuned_bloom(ctx, sp)
+        d = dim_desc(cols["keys"], n_dim, n_dim, attr=cols["attr"], predicate=single_join_pred(thr), bloom=bloom)
+        out.append({"dim": ctx.build_dim(d), "threshold": thr, "bloom": bloom})
+    return cols, out
+
+
+def serve_pool(ctx, seed, n_pool, n_dim=1 << 22, torch_device="cuda"):
+    spec = {"n_dim": n_dim, "n_fact": n_pool, "seed": int(seed) + 1, "threshold": 0}
+    return batch_instance_device(ctx, spec, torch_device)
+
+
+def serve_requests(seed, n_req, n_templates, n_pool, log_lo=18, log_span=8):
+    """[(template, offset, rows)]: offsets are multiples of 4 (16-byte aligned slices)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_req):
+        rows = int(np.floor(2.0 ** (log_lo + log_span * rng.random())))
+        rows = min(rows, n_pool)
+        off = int(rng.integers(0, (n_pool - rows) // 4 + 1)) * 4
+        out.append((int(rng.integers(0, n_templates)), off, rows))
+    return out
+
+
+def serve_desc(pool, tmpl, offset, rows):
+    fk = pool["fk"].data_ptr() + 4 * offset
+    meas = pool["meas"].data_ptr() + 4 * offset
+    return star_desc([fk], meas, rows, None, (0,), method="lip", resident=[tmpl["dim"]])
+
+
 __all__ = [n for n in dir() if not n.startswith("_")] + ["BLOOM_MOD"]

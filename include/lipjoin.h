@@ -260,6 +260,55 @@ lj_status lj_batch_create(lj_ctx* ctx, const lj_star_desc* descs, int32_t n, con
 lj_status lj_batch_run(lj_batch* b, lj_star_result* results);
 void lj_batch_destroy(lj_batch* b);
 
+/* ---- serving ---------------------------------------------------------------------
+ * A request loop over device-resident dimensions (lj_dim_build): callers
+ * submit star joins (every dims[d] resident, fact columns anywhere in device
+ * memory, e.g. slices of a resident pool) from any thread and poll for the
+ * results.  The loop runs on its own host thread and drives the context
+ * alone (do not call other lj_* functions on the context until lj_serve_stop).
+ * Continuous batching: requests that arrive while launches are in flight
+ * wait; whenever one of max_inflight launch slots is free, every waiting
+ * request that the HBM budget admits (up to max_batch) goes into ONE launch
+ * (the batched probe, on the slot's own stream, so launches overlap on the
+ * device).  The budget bounds the per-request device memory in flight
+ * (statistics rows, group accumulators, result regions).  Results are
+ * identical to lj_star_join of the same query.  No reference counterpart:
+ * the reference runs one query per process invocation. */
+typedef struct lj_server lj_server;
+
+typedef struct {
+  uint64_t hbm_budget;     /* bytes of per-request device memory in flight; 0: unlimited */
+  int32_t max_batch;       /* requests per launch; 0: 64 */
+  int32_t max_inflight;    /* launches in flight, each on its own stream; 0: 2 */
+  int64_t groups_cap;      /* result groups kept per request; 0: 64 */
+  int64_t reserved[2];
+} lj_serve_cfg;
+
+typedef struct {
+  uint64_t submitted;
+  uint64_t completed;
+  uint64_t rejected;       /* failed admission or validation */
+  uint64_t launches;
+  uint64_t bytes_in_flight;
+  uint64_t peak_bytes_in_flight;
+  uint64_t max_batch_seen;
+  uint64_t reserved;
+} lj_serve_stats;
+
+lj_status lj_serve_start(lj_ctx* ctx, const lj_serve_cfg* cfg, lj_server** out);
+/* Queues a request (the descriptor is copied); *id identifies it for poll. */
+lj_status lj_serve_submit(lj_server* srv, const lj_star_desc* q, uint64_t* id);
+/* Waits up to timeout_ms (0: do not wait, < 0: forever) for request id.
+ * LJ_ETIMEOUT while it is not done.  When done, fills out like lj_star_join
+ * (at most out->groups_cap groups, ascending g) with time_query_ms = submit to
+ * result ready (host clock) and time_probe_ms = the device time of its
+ * launch, returns the request's status and forgets the request. */
+lj_status lj_serve_poll(lj_server* srv, uint64_t id, lj_star_result* out, int32_t timeout_ms);
+lj_status lj_serve_get_stats(lj_server* srv, lj_serve_stats* out);
+/* Finishes every queued request, stops the loop and frees the server
+ * (results not yet polled are dropped). */
+lj_status lj_serve_stop(lj_server* srv);
+
 /* ---- Bloom filter autotuner ------------------------------------------------------
  * Picks the Bloom filter (hash, layout, k, bits) of a LIP join on one
  * dimension: n_selected keys pass the dimension predicate, the fact side has
