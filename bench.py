@@ -105,6 +105,83 @@ def bench_batch(ctx, n_inst, reps):
     return out
 
 
+def _pct(xs, p):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(round(p / 100.0 * (len(xs) - 1))))] if xs else None
+
+
+def bench_serve(ctx, n_req, fractions=(0.25, 0.5, 0.75, 0.9, 1.1, 1.5, 2.0)):
+    """Config 5: Poisson arrivals of single-join requests (fact slices of
+    2^18..2^26 rows from a resident 2^26-row pool, one of 8 predicate templates
+    over a resident 2^22-key dimension) at rates that are fractions of the
+    measured one-at-a-time capacity; p50/p99 latency (arrival to result ready)
+    and rows/s for the serving loop and for one-query-at-a-time."""
+    import numpy as np
+    import torch
+    from lipjoin import workloads as W
+    seed = W.seed_of(5)
+    n_pool = 1 << 26
+    cols, tmpls = W.serve_templates(ctx, seed)
+    pool = W.serve_pool(ctx, seed, n_pool)
+    torch.cuda.synchronize()
+    reqs = W.serve_requests(seed, n_req, len(tmpls), n_pool)
+    descs = [W.serve_desc(pool, tmpls[t], o, n) for (t, o, n) in reqs]
+    rows = [n for (_, _, n) in reqs]
+    ref = [ctx.star_join(d) for d in descs[:64]]      # warm-up; reference results
+    t = []
+    for d in descs[:min(n_req, 500)]:
+        t0 = time.perf_counter()
+        ctx.star_join(d)
+        t.append(time.perf_counter() - t0)
+    service = statistics.fmean(t)
+    capacity = 1.0 / service
+    out = {"requests_per_rate": n_req, "serial_service_ms": service * 1e3, "serial_capacity_req_s": capacity,
+           "rates": []}
+    rng = np.random.default_rng(seed + 7)
+    for f in fractions:
+        lam = f * capacity
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
+        # serving loop
+        srv = ctx.serve(max_batch=64, max_inflight=2)
+        lag = []
+        t0 = time.perf_counter()
+        ids = []
+        for i, d in enumerate(descs):
+            while time.perf_counter() - t0 < arrivals[i]:
+                pass
+            lag.append(time.perf_counter() - t0 - arrivals[i])
+            ids.append(srv.submit(d))
+        res = [srv.poll(i, timeout_ms=120000) for i in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.stop()
+        lat = [r["time_query_ms"] + 1e3 * lg for r, lg in zip(res, lag)]
+        exact = all((r["sum_total"], r["n_rows"]) == (a["sum_total"], a["n_rows"]) for r, a in zip(res, ref))
+        # one query at a time, same arrivals
+        lat_s = []
+        t0 = time.perf_counter()
+        for i, d in enumerate(descs):
+            while time.perf_counter() - t0 < arrivals[i]:
+                pass
+            ctx.star_join(d)
+            lat_s.append((time.perf_counter() - t0 - arrivals[i]) * 1e3)
+        wall_s = time.perf_counter() - t0
+        e = {"fraction_of_serial_capacity": f, "rate_req_s": lam,
+             "served_p50_ms": _pct(lat, 50), "served_p99_ms": _pct(lat, 99),
+             "served_rows_per_s": sum(rows) / wall, "launches": st["launches"],
+             "mean_requests_per_launch": st["completed"] / max(st["launches"], 1),
+             "serial_p50_ms": _pct(lat_s, 50), "serial_p99_ms": _pct(lat_s, 99),
+             "serial_rows_per_s": sum(rows) / wall_s, "bit_exact": exact,
+             "submit_lag_p99_ms": _pct(lag, 99) * 1e3}
+        out["rates"].append(e)
+        log(f"serve x{f}: served p50 {e['served_p50_ms']:.3f} p99 {e['served_p99_ms']:.3f} ms "
+            f"({e['mean_requests_per_launch']:.1f} req/launch) | serial p50 {e['serial_p50_ms']:.3f} "
+            f"p99 {e['serial_p99_ms']:.3f} ms")
+    for tm in tmpls:
+        tm["dim"].release()
+    return out
+
+
 def cpu_baseline(n_fact, seed, threads):
     """Oracle (C restatement of the reference semantics) on the host cores:
     the LIP plan in join order S->C->D, whole 120M-row input, median of 5
@@ -141,6 +218,8 @@ def main():
     ap.add_argument("--no-plans", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=64,
                     help="config-4 batch size (0 skips the batched measurement)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="config-5 requests per arrival rate (0 skips the serving measurement)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -261,6 +340,8 @@ def main():
         del cols, descs
         torch.cuda.empty_cache()
         out["batched"] = bench_batch(ctx, args.batch_instances, reps=5)
+    if args.serve_requests > 0:
+        out["serving"] = bench_serve(ctx, args.serve_requests)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")

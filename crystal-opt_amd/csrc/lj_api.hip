@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -1556,6 +1557,7 @@ struct lj_server {
   std::condition_variable cv_work;   // the loop waits for requests
   std::condition_variable cv_done;   // pollers wait for results
   std::deque<ServeReq> pending;
+  std::atomic<uint64_t> n_pending{0};  // pending.size(), read without the lock by the idle spin
   std::unordered_map<uint64_t, ServeDone> done;
   uint64_t next_id = 1;
   bool stopping = false;
@@ -1630,6 +1632,11 @@ static void serve_loop(lj_server* S) {
     std::vector<ServeReq> take;
     int slot = -1;
     uint64_t take_bytes = 0;
+    if (inflight.empty()) {
+      // idle: spin briefly for the next request before sleeping on the condition
+      const Clock::time_point until = Clock::now() + std::chrono::microseconds(200);
+      while (S->n_pending.load(std::memory_order_acquire) == 0 && Clock::now() < until) std::this_thread::yield();
+    }
     {
       std::unique_lock<std::mutex> lk(S->mu);
       if (inflight.empty() && S->pending.empty()) {
@@ -1648,6 +1655,7 @@ static void serve_loop(lj_server* S) {
                          nullptr, Clock::now());
             S->stats.rejected++;
             S->pending.pop_front();
+            S->n_pending.store(S->pending.size(), std::memory_order_release);
             S->cv_done.notify_all();
             continue;
           }
@@ -1656,6 +1664,7 @@ static void serve_loop(lj_server* S) {
           take.push_back(rq);
           S->pending.pop_front();
         }
+        S->n_pending.store(S->pending.size(), std::memory_order_release);
         if (!take.empty()) {
           S->slot_busy[(size_t)slot] = true;
           S->stats.bytes_in_flight += take_bytes;
@@ -1691,7 +1700,7 @@ static void serve_loop(lj_server* S) {
       }
       continue;
     }
-    if (!inflight.empty()) std::this_thread::sleep_for(std::chrono::microseconds(2));
+    if (!inflight.empty()) std::this_thread::yield();  // poll the launches in flight
   }
 }
 
@@ -1740,6 +1749,7 @@ extern "C" lj_status lj_serve_submit(lj_server* S, const lj_star_desc* q, uint64
     rq.id = S->next_id++;
     *id = rq.id;
     S->pending.push_back(rq);
+    S->n_pending.store(S->pending.size(), std::memory_order_release);
     S->stats.submitted++;
   }
   S->cv_work.notify_one();
