@@ -105,6 +105,65 @@ def bench_batch(ctx, n_inst, reps):
     return out
 
 
+def bench_single(ctx, reps=5):
+    """Configs 1 and 2: the plain FK-PK join (2^20-row dimension, 2^24 fact
+    rows, attr < 10 of U[0,99], no filter) and the Bloom join (2^22 / 2^28,
+    selectivity 1-50 %, MULT filters 2^16..2^26 bits x k 1..3 and the
+    autotuner's choice, LIP against naive).  Probe times from HIP events;
+    algorithmic bytes = 4 B x fact rows + 128 B x measure lines touched."""
+    import torch
+    from lipjoin import workloads as W
+
+    def timed(dfk, dm, nf, desc, method):
+        r = None
+        ts = []
+        for i in range(reps + 1):
+            r = ctx.hash_join(dfk, dm, nf, desc, method=method)
+            if i:
+                ts.append(r["time_probe"])
+        st = r["stats"]
+        b = 4 * nf + 128 * sum(st["lines_in"][1:2])
+        ms = statistics.median(ts)
+        return {"probe_ms": ms, "rows_per_s": nf / (ms * 1e-3), "bytes_alg": b,
+                "hbm_frac": b / (ms * 1e-3) / HBM_PEAK, "n_match": r["n_match"], "sum": r["sum_measure"]}
+
+    out = {}
+    seed, nd, nf = W.seed_of(1), 1 << 20, 1 << 24
+    dk, da, dfk, dm = W.single_join_device(ctx, seed, nd, nf, 99)
+    desc = L_dim(dk, nd, da, W.single_join_pred(10), None)
+    out["config1"] = timed(dfk, dm, nf, desc, "naive")
+    del dk, da, dfk, dm
+    seed, nd, nf = W.seed_of(2), 1 << 22, 1 << 28
+    dk, da, dfk, dm = W.single_join_device(ctx, seed, nd, nf, 9999)
+    torch.cuda.synchronize()
+    rows = []
+    for sel in (1, 5, 10, 25, 50):
+        thr = 100 * sel
+        pred = W.single_join_pred(thr)
+        e = {"selectivity_pct": sel, "naive": timed(dfk, dm, nf, L_dim(dk, nd, da, pred, None), "naive")}
+        best = None
+        for lg in (16, 18, 20, 22, 24, 26):
+            for k in (1, 2, 3):
+                r = timed(dfk, dm, nf, L_dim(dk, nd, da, pred, ("mult", 1 << lg, k, 0)), "lip")
+                if best is None or r["probe_ms"] < best[1]["probe_ms"]:
+                    best = (f"mult k={k} 2^{lg}", r)
+        e["lip_best"] = dict(best[1], bloom=best[0])
+        sp = {"n_dim": nd, "threshold": thr}
+        bl, ch = W.tuned_bloom(ctx, sp)
+        e["lip_tuned"] = dict(timed(dfk, dm, nf, L_dim(dk, nd, da, pred, bl), "lip"), bloom=str(bl))
+        assert e["lip_tuned"]["sum"] == e["naive"]["sum"] == e["lip_best"]["sum"]
+        rows.append(e)
+        log(f"config2 s={sel}%: naive {e['naive']['probe_ms']:.3f} ms, LIP best {e['lip_best']['probe_ms']:.3f} "
+            f"({best[0]}), tuned {e['lip_tuned']['probe_ms']:.3f} ({bl})")
+    out["config2"] = rows
+    return out
+
+
+def L_dim(keys, n, attr, pred, bloom):
+    import lipjoin as L
+    return L.dim_desc(keys, n, n, attr=attr, predicate=pred, bloom=bloom)
+
+
 def _pct(xs, p):
     xs = sorted(xs)
     return xs[min(len(xs) - 1, int(round(p / 100.0 * (len(xs) - 1))))] if xs else None
@@ -220,6 +279,7 @@ def main():
                     help="config-4 batch size (0 skips the batched measurement)")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="config-5 requests per arrival rate (0 skips the serving measurement)")
+    ap.add_argument("--no-single", action="store_true", help="skip the config 1/2 single-join measurements")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -342,6 +402,8 @@ def main():
         out["batched"] = bench_batch(ctx, args.batch_instances, reps=5)
     if args.serve_requests > 0:
         out["serving"] = bench_serve(ctx, args.serve_requests)
+    if not args.no_single:
+        out["single_join"] = bench_single(ctx)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")

@@ -306,6 +306,8 @@ struct WaveCtx {
   int32_t p_row;    // lane's entry: row | bit 31 = stage 1 done
   int32_t p_x0, p_x1;
   int32_t p_k;      // key gathered for the entry's first remaining stage
+  uint64_t p_ht;    // one dimension, LIP: the hash-table slot of the key (gathered)
+  int32_t p_m;      // one dimension: the measure of the row (gathered)
   __device__ __forceinline__ int ld(int s) const { return s == 0 ? ld0 : s == 1 ? ld1 : s == 2 ? ld2 : ld3; }
   __device__ __forceinline__ int32_t last(int s) const { return s == 1 ? last1 : s == 2 ? last2 : last3; }
   __device__ __forceinline__ void set_last(int s, int32_t x) {
@@ -324,7 +326,9 @@ struct WaveCtx {
 // An entry is (row, x0, x1): x is the key (LIP: the hash probes come last)
 // or the payload (naive) of the stage's dimension; x0 is stage 0's, and x1
 // stage 1's when bit 31 of row says the row passed stage 1 already (the
-// survivors of a dense segment).
+// survivors of a dense segment).  With one dimension, bit 31 says that x1 is
+// the row's measure (read from the streamed measure column).
+template <bool LIP>
 __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int cnt) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -342,6 +346,21 @@ __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int 
     const int32_t* col = first == 1 ? a.dim[w.ld1].fk : a.dim[w.ld2].fk;
     k = col[(uint32_t)(r & 0x7FFFFFFF)];
   }
+  // one dimension: the rows only need their hash slot (LIP) and measure; both
+  // gathers go out now
+  uint64_t ht = 0;
+  int32_t m = 0;
+  if (alive && w.nd == 1) {
+    const uint32_t row = (uint32_t)(r & 0x7FFFFFFF);
+    m = r < 0 ? x1 : a.measure[row];
+    if (LIP) {
+      const ProbeDim& D = a.dim[0];
+      const int64_t sl = pht_slot(x0, D.ht_len, D.keys_min);
+      if (sl >= 0) ht = *reinterpret_cast<const uint64_t*>(D.ht + 2 * sl);
+    }
+  }
+  w.p_ht = ht;
+  w.p_m = m;
   w.p_cnt = cnt;
   w.p_row = r;
   w.p_x0 = x0;
@@ -364,7 +383,7 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
   if (alive) {
     if (LIP) key.set(w.ld0, w.p_x0);
     else pay.set(w.ld0, w.p_x0);
-    if (done1) {
+    if (done1 && w.nd >= 2) {
       if (LIP) key.set(w.ld1, w.p_x1);
       else pay.set(w.ld1, w.p_x1);
     }
@@ -396,21 +415,27 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
   }
   if (LIP) {
     st.v[ST_BLOOM_PASS] += __popcll(__ballot(alive));
+    if (nd == 1) {
+      // the slot gathered by issue_batch (pht2_probe: a nonzero slot is a hit)
+      alive = alive && w.p_ht != 0;
+      pay.set(0, (int32_t)(uint32_t)(w.p_ht >> 32));
+    } else {
 #pragma unroll
-    for (int t = 0; t < kMaxDims; t++) {
-      if (t >= nd) continue;
-      const int e = a.order[t];
-      if (alive) {
-        int32_t p = 0;
-        alive = pht_probe(a.dim[e], key.get(e), p);
-        pay.set(e, p);
+      for (int t = 0; t < kMaxDims; t++) {
+        if (t >= nd) continue;
+        const int e = a.order[t];
+        if (alive) {
+          int32_t p = 0;
+          alive = pht_probe(a.dim[e], key.get(e), p);
+          pay.set(e, p);
+        }
       }
     }
   }
   st.meas_rows += __popcll(__ballot(alive));
   st.meas_lines += batch_lines(alive, row, w.last_meas);
   int32_t m = 0;
-  if (alive) m = a.measure[(uint32_t)row];
+  if (alive) m = nd == 1 ? w.p_m : a.measure[(uint32_t)row];
   st.v[ST_OOB] += group_add(a, gt, alive, pay, m);
   w.p_cnt = 0;
 }
@@ -422,7 +447,7 @@ __device__ __forceinline__ void rotate_batch(const ProbeArgs& a, const uint32_t*
                                              WaveStats& st) {
   complete_batch<LIP>(a, lds, gt, w, st);
   const int take = w.qcount < kQueue ? w.qcount : kQueue;
-  issue_batch(a, w, take);
+  issue_batch<LIP>(a, w, take);
   const int rest = w.qcount - take;
   if (rest > 0) {
     // reads from [kQueue, kQueue + rest), writes to [0, rest): no overlap, and
@@ -683,8 +708,17 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
       qx1[j] = 0;
     }
     if (cnt > kDenseMin || have_eager) {
-      uint32_t qm = 0;
-      if (!process_dense<LIP>(a, lds, gt, w, row0, kv, pv, m, ev, have_eager, st, qm, qx0, qx1)) continue;
+      // with two or more dimensions stage 1 runs here on the streamed column;
+      // with one, every survivor goes to the queue (its hash-slot gather is
+      // issued a batch ahead) carrying its measure when the column streams
+      uint32_t qm = m;
+      const bool d1 = w.nd >= 2 || have_eager;
+      if (w.nd == 1 && have_eager) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) qx1[j] = vget(ev, j);
+      }
+      if (w.nd >= 2 && !process_dense<LIP>(a, lds, gt, w, row0, kv, pv, m, ev, have_eager, st, qm, qx0, qx1))
+        continue;
       const int cq = popc_ballot4(qm);
       if (cq == 0) continue;
       // the stage-1 survivors (up to 4 per lane) go in up to four parts of
@@ -696,7 +730,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
         const int c = parts == 1 ? cq : popc_ballot4(pm);
         if (c == 0) continue;
         if (w.qcount + c > kQueueCap) rotate_batch<LIP>(a, lds, gt, w, st);
-        enqueue(w, row0, qx0, qx1, pm, true, c);
+        enqueue(w, row0, qx0, qx1, pm, d1, c);
       }
       continue;
     }
@@ -952,7 +986,8 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   w.qcount = 0;
   w.last1 = w.last2 = w.last3 = w.last_meas = -1;
   w.p_cnt = 0;
-  w.p_row = w.p_x0 = w.p_x1 = w.p_k = 0;
+  w.p_row = w.p_x0 = w.p_x1 = w.p_k = w.p_m = 0;
+  w.p_ht = 0;
   w.ld0 = a.load_order[0];
   w.ld1 = a.load_order[1];
   w.ld2 = a.load_order[2];
