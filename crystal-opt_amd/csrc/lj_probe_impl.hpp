@@ -115,19 +115,30 @@ __device__ __forceinline__ uint32_t s0_probe4(const S0& s, const v4i32& k, uint3
   const int32_t kk[4] = {k.x, k.y, k.z, k.w};
   uint32_t ok = 0;
   if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_LDS || K0 == S0_MOD_GL) {
+    // bit h = (key - keys_min) mod bits: word h >> 5, bit h & 31 (a power-of-two
+    // size masks the word index only: the bit index is the low 5 bits either
+    // way).  A key below keys_min is counted out of domain by the caller; its
+    // bit is tested like any other and a hash probe later rejects it.
     uint32_t h[4], wv[4];
+    // a power-of-two size of 2^(5 + wbits) bits: word (x >> 5) & (2^wbits - 1)
+    const uint32_t wbits = (uint32_t)__popc(s.mask) - 5u;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const uint32_t x = (uint32_t)kk[j] - (uint32_t)s.bh.keys_min;
-      if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) h[j] = x & s.mask;
+      if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) h[j] = x;
       else h[j] = fastmod_u32(x, s.bh.mod);
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++) wv[j] = s.bf[h[j] >> 5];
+    for (int j = 0; j < 4; j++) {
+      if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL)
+        wv[j] = s.bf[__builtin_amdgcn_ubfe(h[j], 5u, wbits)];
+      else
+        wv[j] = s.bf[h[j] >> 5];
+    }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const uint32_t bit = (wv[j] >> (h[j] & 31)) & 1u;
-      ok |= (bit & (uint32_t)(kk[j] >= s.bh.keys_min)) << j;
+      // v_bfe_u32 reads bits [4:0] of the offset: bit h & 31 of the word
+      ok |= __builtin_amdgcn_ubfe(wv[j], h[j], 1u) << j;
       pay[j] = 0;
     }
   } else if constexpr (K0 == S0_PHT) {
@@ -670,9 +681,10 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
 #pragma unroll
       for (int j = 0; j < 4; j++) valid4 |= (uint32_t)(row0 + j < tc.n) << j;
     }
-    const uint32_t ge4 = (uint32_t)(k[v].x >= s0.kmin) | ((uint32_t)(k[v].y >= s0.kmin) << 1) |
-                         ((uint32_t)(k[v].z >= s0.kmin) << 2) | ((uint32_t)(k[v].w >= s0.kmin) << 3);
-    oob += __popc(valid4 & ~ge4);
+    // keys below keys_min: counted here (wave-wide, on the scalar unit)
+    const int32_t km = s0.kmin;
+    st.v[ST_OOB] += __popcll(__ballot((valid4 & 1u) && k[v].x < km)) + __popcll(__ballot((valid4 & 2u) && k[v].y < km)) +
+                    __popcll(__ballot((valid4 & 4u) && k[v].z < km)) + __popcll(__ballot((valid4 & 8u) && k[v].w < km));
     mask[v] = s0_probe4<K0>(s0, k[v], valid4, pay[v]);
   }
   const uint32_t nvalid = CHECKED ? (tc.n - base < TILE ? tc.n - base : TILE) : TILE;
