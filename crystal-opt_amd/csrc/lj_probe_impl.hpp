@@ -762,7 +762,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
 template <int VEC>
 __device__ __forceinline__ void sample_tile(const ProbeArgs& a, const uint32_t* lds_filter, WaveCtx& w,
                                             WaveStats& st, unsigned long long* lds_samp, uint32_t base,
-                                            uint32_t lane4, int64_t n64, bool& eager) {
+                                            uint32_t lane4, int64_t n64, const v4i32 (&pre)[kMaxDims], bool& eager) {
   uint32_t cnt[kMaxDims] = {0, 0, 0, 0};
   uint32_t n_valid = 0;
 #pragma unroll
@@ -771,11 +771,11 @@ __device__ __forceinline__ void sample_tile(const ProbeArgs& a, const uint32_t* 
     uint32_t valid = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++) valid |= (uint32_t)((int64_t)row0 + j < n64) << j;
-    // one column at a time (a rolled loop keeps the register peak of this
-    // once-per-wave step below the streaming loop's)
-#pragma unroll 1
-    for (int d = 0; d < w.nd; d++) {
-      const v4i32 kk = WaveLoadVec4<false>(a.dim[d].fk, row0, n64);
+    // segment 0's columns were loaded while the filters staged (pre)
+#pragma unroll
+    for (int d = 0; d < kMaxDims; d++) {
+      if (d >= w.nd) continue;
+      const v4i32 kk = v == 0 ? pre[d] : WaveLoadVec4<false>(a.dim[d].fk, row0, n64);
       uint32_t m = 0;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
@@ -960,6 +960,24 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
 
   if (bid == 0 && threadIdx.x == 0) a.out_hdr[kClockSlot + 1] = (long long)wall_clock64();
 
+  // the wave's first tile of its first column is read while the filters stage
+  constexpr uint32_t TILE0 = (uint32_t)VEC * kSegRows;
+  const uint32_t gw0 = bid * NW + wave;
+  const int64_t n_first = a.n;
+  v4i32 k_first[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; v++)
+    k_first[v] = (int64_t)gw0 * TILE0 < n_first && !a.adaptive
+                     ? WaveLoadVec4<true>(a.dim[a.load_order[0]].fk, (int64_t)(gw0 * TILE0 + v * kSegRows + lane * 4), n_first)
+                     : v4i32{0, 0, 0, 0};
+  // adaptive: every column of the sampled segment
+  v4i32 k_samp[kMaxDims];
+#pragma unroll
+  for (int d = 0; d < kMaxDims; d++)
+    k_samp[d] = K0 != S0_PHT && a.adaptive && d < a.n_dims && (int64_t)gw0 * TILE0 < n_first
+                    ? WaveLoadVec4<false>(a.dim[d].fk, (int64_t)(gw0 * TILE0 + lane * 4), n_first)
+                    : v4i32{0, 0, 0, 0};
+
   // ---- stage the LDS-resident filters, clear the group table
   for (int d = 0; d < a.n_dims; d++) {
     const ProbeDim& D = a.dim[d];
@@ -1018,7 +1036,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   const uint32_t n_tiles = (n_rows + TILE - 1) / TILE;
   if constexpr (K0 != S0_PHT) {
     if (a.adaptive && gw < n_tiles)
-      sample_tile<VEC>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, eager);
+      sample_tile<VEC>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, k_samp, eager);
   }
 
   // ---- streaming loop over tiles -------------------------------------------------
@@ -1052,7 +1070,9 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       v4i32 k[VEC], ke[VEC];
 #pragma unroll
       for (int v = 0; v < VEC; v++) {
-        k[v] = WaveLoadVec4<true>(s0.fk, (int64_t)(base + v * kSegRows + tc.lane4), n64);
+        // the first tile of a static order was loaded before the staging
+        k[v] = pass == 0 && !a.adaptive ? k_first[v]
+                                        : WaveLoadVec4<true>(s0.fk, (int64_t)(base + v * kSegRows + tc.lane4), n64);
         ke[v] = k[v];
       }
       const int tot = tile_step<VEC, K0, true>(a, s0, lds_filter, gt, w, st, tc, base, k, ke, false, oob);
