@@ -117,6 +117,16 @@ static lj_status fail(lj_ctx* c, lj_status s, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail(c, LJ_EHIP, "%s: %s", #expr, hipGetErrorString(e_));        \
   } while (0)
 
+// Wait for an event by polling: a query is a few hundred microseconds of
+// device work, and a blocking wait's wake-up would add tens of microseconds.
+static hipError_t spin_wait(hipEvent_t e) {
+  for (;;) {
+    const hipError_t q = hipEventQuery(e);
+    if (q != hipErrorNotReady) return q;
+    std::this_thread::yield();
+  }
+}
+
 // ---------------------------------------------------------- caching allocator --
 static size_t round_block(size_t bytes) {
   size_t b = 256;
@@ -878,7 +888,7 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
   // the built tables go back to the zero pool now: their clears queue behind the copy
   cleanup();
-  HIP_TRY(c, hipEventSynchronize(c->ev[2]));
+  HIP_TRY(c, spin_wait(c->ev[2]));
   const int64_t n_touched = hdr[0];
   if (n_touched > (int64_t)c->touched_cap)
     return fail(c, LJ_EUNSUPPORTED, "%lld groups were hit; at most %u can be listed", (long long)n_touched, c->touched_cap);
@@ -1375,7 +1385,7 @@ extern "C" lj_status lj_batch_run(lj_batch* b, lj_star_result* results) {
   HIP_TRY(c, hipSetDevice(c->device));
   lj_status s = batch_launch(b);
   if (s != LJ_OK) return s;
-  HIP_TRY(c, hipEventSynchronize(b->e1));
+  HIP_TRY(c, spin_wait(b->e1));
   return batch_collect(b, results);
 }
 

@@ -194,6 +194,23 @@ class Context:
         return out
 
     # -- star join (runQuery) ---------------------------------------------------------
+    def star_join_raw(self, desc, groups_cap=4096):
+        """lj_star_join into a reused result structure, without building the
+        Python result (the C-ABI call alone); star_join_result() reads it."""
+        if getattr(self, "_gbuf_cap", 0) < groups_cap:
+            self._gbuf = (_abi.Group * groups_cap)()
+            self._gbuf_cap = groups_cap
+        if getattr(self, "_raw_res", None) is None:
+            self._raw_res = _abi.StarResult()
+        r = self._raw_res
+        r.groups, r.groups_cap = C.cast(self._gbuf, C.POINTER(_abi.Group)), groups_cap
+        self._raw_nd = desc.n_dims
+        return self._check(lib().lj_star_join(self.h, C.byref(desc), C.byref(r)))
+
+    def star_join_result(self):
+        r = self._raw_res
+        return _star_result_dict(r, self._gbuf, self._raw_nd, r.groups_cap)
+
     def star_join(self, desc, groups_cap=4096, allow_domain=False):
         # the host result buffer is kept and reused across calls
         if getattr(self, "_gbuf_cap", 0) < groups_cap:
