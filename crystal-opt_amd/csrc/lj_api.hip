@@ -50,6 +50,9 @@ struct lj_dim {
   float time_build_ms = 0.f;
 };
 
+constexpr int kEvents = 4;
+constexpr int kEvDone = 1;
+
 struct lj_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -63,7 +66,9 @@ struct lj_ctx {
   std::unordered_map<void*, size_t> live;
   int n_cu = 0;
   int max_lds = 65536;
-  hipEvent_t ev[6]{};
+  // ev[0] before the build, ev[3] between build and probe, ev[1] (kEvDone)
+  // after the probe (which writes the result to pinned host memory itself)
+  hipEvent_t ev[kEvents]{};
   // group state (kept zero between queries by the finalize kernel)
   int64_t g_cap = 0;
   unsigned long long* gsum = nullptr;
@@ -79,6 +84,7 @@ struct lj_ctx {
   long long* stats = nullptr;
   int stats_cap = 0;
   void* host_buf = nullptr;  // pinned: header + first kHostGroups groups
+  void* host_buf_dev = nullptr;  // the same memory, as the device addresses it
   // tuning (environment overrides for sweeps: LJ_PROBE_WG, LJ_PROBE_VEC, LJ_LDS_POLICY)
   int probe_wg = 0;
   int probe_vec = 0;
@@ -333,10 +339,18 @@ extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* 
       return bail(fail(c, LJ_EHIP, "hipStreamCreate failed"));
     c->own_stream = true;
   }
-  for (auto& ev : c->ev)
-    if (hipEventCreate(&ev) != hipSuccess) return bail(fail(c, LJ_EHIP, "hipEventCreate failed"));
+  // ev[1] follows the probe and is the one the host waits for: it keeps the
+  // system-scope release (the result is in pinned host memory).  The others only time phases; without the
+  // system fence, recording them does not write back and invalidate the
+  // caches between the build and the probe (the probe would re-read the
+  // filters and tables it finds in L2 from memory).
+  for (int i = 0; i < kEvents; i++)
+    if (hipEventCreateWithFlags(&c->ev[i], i == kEvDone ? hipEventDefault : hipEventDisableSystemFence) != hipSuccess)
+      return bail(fail(c, LJ_EHIP, "hipEventCreate failed"));
   if (hipHostMalloc(&c->host_buf, kHdrBytes + sizeof(GroupOut) * kHostGroups) != hipSuccess)
     return bail(fail(c, LJ_EHIP, "hipHostMalloc failed"));
+  if (hipHostGetDevicePointer(&c->host_buf_dev, c->host_buf, 0) != hipSuccess)
+    return bail(fail(c, LJ_EHIP, "hipHostGetDevicePointer failed"));
   c->probe_wg = env_int("LJ_PROBE_WG", 0);
   c->probe_vec = env_int("LJ_PROBE_VEC", 0);
   c->lds_policy = env_int("LJ_LDS_POLICY", -1);
@@ -864,6 +878,11 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   a.out_hdr = c->out_hdr;
   a.out = c->out;
   a.out_cap = c->touched_cap;
+  // the header and the first groups go straight to the pinned host buffer
+  const int64_t first = std::min<int64_t>(kFirstGroups, c->touched_cap);
+  a.host_hdr = static_cast<long long*>(c->host_buf_dev);
+  a.host_out = reinterpret_cast<GroupOut*>(static_cast<char*>(c->host_buf_dev) + kHdrBytes);
+  a.host_groups = first;
 
   // ---- two launches: the build of the query's own dimensions, then the probe
   // (which also collects the result groups in its last workgroup)
@@ -877,18 +896,15 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   }
   HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
   HIP_TRY(c, launch_probe(cfg, a, (int)grid, p.lds_bytes, c->stream));
-  HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev[kEvDone], c->stream));
 
-  // ---- results
-  // one copy: header (groups hit, statistics, dimension counters) and the first groups
+  // ---- results: the header (groups hit, statistics, dimension counters) and
+  // the first groups are in the pinned host buffer once the probe is done
   long long* hdr = static_cast<long long*>(c->host_buf);
   GroupOut* hg = reinterpret_cast<GroupOut*>(static_cast<char*>(c->host_buf) + kHdrBytes);
-  const int64_t first = std::min<int64_t>(kFirstGroups, c->touched_cap);
-  HIP_TRY(c, hipMemcpyAsync(hdr, c->out_buf, kHdrBytes + sizeof(GroupOut) * first, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
-  // the built tables go back to the zero pool now: their clears queue behind the copy
+  // the built tables go back to the zero pool now: their clears queue behind the probe
   cleanup();
-  HIP_TRY(c, spin_wait(c->ev[2]));
+  HIP_TRY(c, spin_wait(c->ev[kEvDone]));
   const int64_t n_touched = hdr[0];
   if (n_touched > (int64_t)c->touched_cap)
     return fail(c, LJ_EUNSUPPORTED, "%lld groups were hit; at most %u can be listed", (long long)n_touched, c->touched_cap);

@@ -99,6 +99,11 @@ struct ProbeArgs {
   GroupOut* out;
   int64_t out_cap;
   const unsigned long long* dim_counters[kMaxDims];
+  // if set (a single query): the header and the first host_groups groups are
+  // also written to this pinned host memory, so no copy follows the probe
+  long long* host_hdr;
+  GroupOut* host_out;
+  int64_t host_groups;
 };
 
 

@@ -606,7 +606,8 @@ __device__ __forceinline__ void load_tile(const int32_t* col, uint32_t base, uin
 // Result collection by one workgroup (the last to finish the probe): copy the
 // touched groups out in touch order, reset their dense accumulators to zero
 // for the next query, reduce the per-workgroup statistics.  `part` is LDS
-// scratch of WG long longs.
+// scratch of WG long longs.  With host_hdr set, the header and the first
+// host_groups groups also go straight to pinned host memory.
 template <int WG>
 __device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* part, uint32_t nblk) {
   constexpr int ROWS = WG / kStats;
@@ -630,6 +631,7 @@ __device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* pa
       o.prod = a.gprod[g];
       for (int d = 0; d < kMaxDims; d++) o.pay[d] = a.gpay[(int64_t)g * kMaxDims + d];
       a.out[i] = o;
+      if (a.host_hdr && (int64_t)i < a.host_groups) a.host_out[i] = o;
     }
     a.gsum[g] = 0;
     a.gcount[g] = 0;
@@ -641,14 +643,19 @@ __device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* pa
     else
       for (int r = 0; r < ROWS; r++) acc += part[r * kStats + k];
     a.out_hdr[1 + k] = acc;
+    if (a.host_hdr) a.host_hdr[1 + k] = acc;
   }
-  if (threadIdx.x == 0) a.out_hdr[0] = n;
+  if (threadIdx.x == 0) {
+    a.out_hdr[0] = n;
+    if (a.host_hdr) a.host_hdr[0] = n;
+  }
   if (threadIdx.x < 2 * kMaxDims) {
     const int d = threadIdx.x / 2;
     long long v = 0;
     if (a.dim_counters[d])
       for (int l = 0; l < kCounterLanes; l++) v += (long long)a.dim_counters[d][2 * l + threadIdx.x % 2];
     a.out_hdr[1 + kStats + threadIdx.x] = v;
+    if (a.host_hdr) a.host_hdr[1 + kStats + threadIdx.x] = v;
   }
   __syncthreads();
   if (threadIdx.x == 0) *a.n_touched = 0;
