@@ -424,29 +424,47 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
       }
     }
   }
+  int32_t m = 0;
   if (LIP) {
-    st.v[ST_BLOOM_PASS] += __popcll(__ballot(alive));
+    const unsigned long long bloom_pass = __ballot(alive);
+    st.v[ST_BLOOM_PASS] += __popcll(bloom_pass);
     if (nd == 1) {
       // the slot gathered by issue_batch (pht2_probe: a nonzero slot is a hit)
       alive = alive && w.p_ht != 0;
       pay.set(0, (int32_t)(uint32_t)(w.p_ht >> 32));
-    } else {
+      m = w.p_m;
+    } else if (bloom_pass != 0) {
+      // After the Bloom chain every key is known: the PHT_2 probes of all
+      // dimensions and the measure gather go out together, one round trip
+      // instead of one per join.  The loads are unconditional (a lane with no
+      // row, or a key outside the table, reads slot 0 / row 0 and drops it),
+      // so no branch separates them.  A row is alive iff every probe hits, as
+      // with the probes taken one after another in join order.
+      // (every slot is loaded, the one of a dimension past nd from dimension
+      // 0's table: no branch on nd between the loads either)
+      uint64_t slot[kMaxDims];
+      bool hit = alive;
+#pragma unroll
+      for (int t = 0; t < kMaxDims; t++) {
+        const bool on = t < nd;
+        const ProbeDim& D = a.dim[on ? t : 0];
+        const int64_t sl = pht_slot(key.get(t), D.ht_len, D.keys_min);
+        hit = hit && (!on || sl >= 0);
+        slot[t] = *reinterpret_cast<const uint64_t*>(D.ht + 2 * (alive && on && sl >= 0 ? sl : 0));
+      }
+      m = a.measure[alive ? (uint32_t)row : 0u];
 #pragma unroll
       for (int t = 0; t < kMaxDims; t++) {
         if (t >= nd) continue;
-        const int e = a.order[t];
-        if (alive) {
-          int32_t p = 0;
-          alive = pht_probe(a.dim[e], key.get(e), p);
-          pay.set(e, p);
-        }
+        hit = hit && slot[t] != 0;
+        pay.set(t, (int32_t)(uint32_t)(slot[t] >> 32));
       }
+      alive = hit;
     }
   }
   st.meas_rows += __popcll(__ballot(alive));
   st.meas_lines += batch_lines(alive, row, w.last_meas);
-  int32_t m = 0;
-  if (alive) m = nd == 1 ? w.p_m : a.measure[(uint32_t)row];
+  if (!LIP && alive) m = nd == 1 ? w.p_m : a.measure[(uint32_t)row];
   st.v[ST_OOB] += group_add(a, gt, alive, pay, m);
   w.p_cnt = 0;
 }
@@ -611,18 +629,35 @@ __device__ __forceinline__ void load_tile(const int32_t* col, uint32_t base, uin
 template <int WG>
 __device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* part, uint32_t nblk) {
   constexpr int ROWS = WG / kStats;
+  // Every load that does not depend on another goes out first (one memory
+  // round trip): the groups-hit count, this thread's entry of the touched
+  // list (read whether or not it is in use), the statistics of all
+  // workgroups and the build counters.  The group values, which need the
+  // touched entries, are the second round trip.  No barrier follows a
+  // global or host store, so nothing waits for a store to complete before
+  // the kernel ends.
   const uint32_t n = __hip_atomic_load(a.n_touched, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t g_first = threadIdx.x < a.touched_cap ? a.touched[threadIdx.x] : 0u;
+  long long dimc = 0;
+  if (threadIdx.x < 2 * kMaxDims) {
+    const int d = threadIdx.x / 2;
+    if (a.dim_counters[d])
+      for (int l = 0; l < kCounterLanes; l++) dimc += (long long)a.dim_counters[d][2 * l + threadIdx.x % 2];
+  }
   {
     const int k = threadIdx.x % kStats, r = threadIdx.x / kStats;
     long long acc = 0;
     if (k != ST_ORDER)
       for (int b = r; b < (int)nblk; b += ROWS) acc += a.stats[(int64_t)b * kStats + k];
+    else if (r == 0)
+      acc = a.stats[ST_ORDER];  // the order used, from workgroup 0
     part[r * kStats + k] = acc;
   }
-  __syncthreads();
+  __syncthreads();  // `part` complete; every thread has read n_touched
+  if (threadIdx.x == 0) *a.n_touched = 0;
   const uint32_t listed = n < a.touched_cap ? n : a.touched_cap;
   for (uint32_t i = threadIdx.x; i < listed; i += WG) {
-    const uint32_t g = a.touched[i];
+    const uint32_t g = i == threadIdx.x ? g_first : a.touched[i];
     if ((int64_t)i < a.out_cap) {
       GroupOut o;
       o.g = g;
@@ -639,26 +674,18 @@ __device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* pa
   }
   for (int k = threadIdx.x; k < kStats; k += WG) {
     long long acc = 0;
-    if (k == ST_ORDER) acc = a.stats[k];
-    else
-      for (int r = 0; r < ROWS; r++) acc += part[r * kStats + k];
+    for (int r = 0; r < ROWS; r++) acc += part[r * kStats + k];
     a.out_hdr[1 + k] = acc;
     if (a.host_hdr) a.host_hdr[1 + k] = acc;
+  }
+  if (threadIdx.x < 2 * kMaxDims) {
+    a.out_hdr[1 + kStats + threadIdx.x] = dimc;
+    if (a.host_hdr) a.host_hdr[1 + kStats + threadIdx.x] = dimc;
   }
   if (threadIdx.x == 0) {
     a.out_hdr[0] = n;
     if (a.host_hdr) a.host_hdr[0] = n;
   }
-  if (threadIdx.x < 2 * kMaxDims) {
-    const int d = threadIdx.x / 2;
-    long long v = 0;
-    if (a.dim_counters[d])
-      for (int l = 0; l < kCounterLanes; l++) v += (long long)a.dim_counters[d][2 * l + threadIdx.x % 2];
-    a.out_hdr[1 + kStats + threadIdx.x] = v;
-    if (a.host_hdr) a.host_hdr[1 + kStats + threadIdx.x] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) *a.n_touched = 0;
 }
 
 struct TileCtx {
@@ -1077,9 +1104,19 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       v4i32 k[VEC], ke[VEC];
 #pragma unroll
       for (int v = 0; v < VEC; v++) {
-        // the first tile of a static order was loaded before the staging
-        k[v] = pass == 0 && !a.adaptive ? k_first[v]
-                                        : WaveLoadVec4<true>(s0.fk, (int64_t)(base + v * kSegRows + tc.lane4), n64);
+        // the first tile of a static order was loaded before the staging; the
+        // adaptive sample loaded segment 0 of every column (copied by value
+        // from the chosen one: no indexed register array)
+        if (pass == 0 && !a.adaptive) {
+          k[v] = k_first[v];
+        } else if (K0 != S0_PHT && pass == 0 && v == 0) {
+          // a select over values (no lvalue conditional: that would index the array)
+          const int l0 = w.ld0;
+          const v4i32 s0v = k_samp[0], s1v = k_samp[1], s2v = k_samp[2], s3v = k_samp[3];
+          k[v] = l0 == 0 ? s0v : l0 == 1 ? s1v : l0 == 2 ? s2v : s3v;
+        } else {
+          k[v] = WaveLoadVec4<true>(s0.fk, (int64_t)(base + v * kSegRows + tc.lane4), n64);
+        }
         ke[v] = k[v];
       }
       const int tot = tile_step<VEC, K0, true>(a, s0, lds_filter, gt, w, st, tc, base, k, ke, false, oob);
