@@ -850,7 +850,22 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
     dims_free(c, p.owned, p.n_owned);
     p.n_owned = 0;
   };
-  if ((s = ensure_groups(c, q->n_groups)) != LJ_OK || (s = plan_probe(c, q, want_product, 0, p)) != LJ_OK) {
+  if ((s = ensure_groups(c, q->n_groups)) != LJ_OK) {
+    cleanup();
+    return s;
+  }
+  // ---- two launches: the build of the query's own dimensions, then the probe
+  // (which also collects the result groups in its last workgroup).  The build
+  // goes out first; the probe is planned while it runs.
+  const bool built_here = p.n_owned > 0;
+  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  if (built_here) {
+    if ((s = dims_launch(c, p.owned, p.n_owned, c->out_hdr + kClockSlot)) != LJ_OK) {
+      cleanup();
+      return s;
+    }
+  }
+  if ((s = plan_probe(c, q, want_product, 0, p)) != LJ_OK) {
     cleanup();
     return s;
   }
@@ -884,16 +899,6 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   a.host_out = reinterpret_cast<GroupOut*>(static_cast<char*>(c->host_buf_dev) + kHdrBytes);
   a.host_groups = first;
 
-  // ---- two launches: the build of the query's own dimensions, then the probe
-  // (which also collects the result groups in its last workgroup)
-  const bool built_here = p.n_owned > 0;
-  HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
-  if (built_here) {
-    if ((s = dims_launch(c, p.owned, p.n_owned, c->out_hdr + kClockSlot)) != LJ_OK) {
-      cleanup();
-      return s;
-    }
-  }
   HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
   HIP_TRY(c, launch_probe(cfg, a, (int)grid, p.lds_bytes, c->stream));
   HIP_TRY(c, hipEventRecord(c->ev[kEvDone], c->stream));

@@ -53,22 +53,28 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
 // unconditional value store) -- build_kernel of src/ops/bloom_join.cu:43-65 and
 // build_hashtable_{s,c,d} of src/ssb-jo/q33/0-lip.cu:114-203.  Every dimension
 // of the query in one launch: each dimension owns a range of workgroups (all
-// dimensions build at once), a thread takes 4 consecutive rows, and the
-// selected / out-of-domain counts reduce per workgroup to one atomic each.
+// dimensions build at once), a thread takes 4 consecutive rows (1 in a
+// dimension of fewer than kBuildSmall rows, so that a small dimension's
+// inserts spread over more workgroups), and the selected / out-of-domain
+// counts reduce per workgroup to one atomic each.
 constexpr int kBuildThreads = 256;
 constexpr int kBuildRows = 4;
+constexpr int64_t kBuildSmall = 1 << 16;
 
-// Workgroup `blk` of dimension B: rows [blk * 1024, (blk + 1) * 1024).
+__host__ __device__ inline int build_rows_per_thread(int64_t rows) { return rows < kBuildSmall ? 1 : kBuildRows; }
+
+// Workgroup `blk` of dimension B: rows [blk * 256 * rpt, (blk + 1) * 256 * rpt).
 __device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
   __shared__ unsigned int cnt[2];
   if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t r0 = (blk * kBuildThreads + threadIdx.x) * kBuildRows;
+  const int rpt = build_rows_per_thread(B.n);
+  const int64_t r0 = (blk * kBuildThreads + threadIdx.x) * rpt;
   uint32_t nsel = 0, noob = 0;
 #pragma unroll
   for (int j = 0; j < kBuildRows; j++) {
     const int64_t r = r0 + j;
-    if (r < B.n) {
+    if (j < rpt && r < B.n) {
       const int32_t key = B.keys[r];
       const bool sel = B.pred.n_steps == 0 || eval_pred(B.pred, B.attr[r]);
       if (sel) {
@@ -124,7 +130,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_multi_kernel(const BuildD
 }
 
 int64_t build_blocks(int64_t rows) {
-  constexpr int64_t rows_per_block = (int64_t)kBuildThreads * kBuildRows;
+  const int64_t rows_per_block = (int64_t)kBuildThreads * build_rows_per_thread(rows);
   return (rows + rows_per_block - 1) / rows_per_block;
 }
 
@@ -136,11 +142,10 @@ hipError_t launch_build_multi(const BuildDim* dims, int n, int64_t total_blocks,
 
 // Assigns the workgroup ranges (block_begin) and launches.
 hipError_t launch_build(BuildArgs& a, hipStream_t s) {
-  constexpr int64_t rows_per_block = (int64_t)kBuildThreads * kBuildRows;
   int64_t blocks = 0;
   for (int d = 0; d < a.n_dims; d++) {
     a.dim[d].block_begin = blocks;
-    blocks += (a.dim[d].n + rows_per_block - 1) / rows_per_block;
+    blocks += build_blocks(a.dim[d].n);
   }
   if (blocks <= 0) return hipSuccess;
   hipLaunchKernelGGL(build_kernel, dim3((unsigned)blocks), dim3(kBuildThreads), 0, s, a);
