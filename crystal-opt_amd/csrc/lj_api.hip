@@ -91,9 +91,13 @@ struct lj_ctx {
   int lds_policy = -1;
   double clock_khz = 100000.0;  // device wall clock (kernel phase timestamps)
   // Bloom autotuner cost model, per fact row, relative to the scan of an LDS filter
-  double tune_scan_gl = 1.5;     // scan with the first filter in L2 instead of LDS
+  // cost model of the Bloom autotuner, per fact row, in units of a scan with
+  // the filter in LDS (defaults: typical values lj_tune_calibrate measures)
+  double tune_scan_gl = 3.0;     // scan with the first filter in L2 instead of LDS (up to kTuneL2Bits)
+  double tune_scan_big = 4.0;    // scan with a larger filter (L2 misses, served from the MALL)
   double tune_probe_extra = 0.2; // each further bit probe of a standard k > 1 filter
-  double tune_surv = 40.0;       // a row that passes the filter (queue, gathers, hash probe)
+  double tune_line = 1.0;        // a 128-byte line of the next column that holds a passing row
+  double tune_gather = 6.0;      // a row that passes the filter (queue, hash-slot gather)
   int tune_calibrated = 0;
   std::map<std::pair<int, size_t>, int> occupancy;  // (config key, LDS bytes) -> workgroups per CU
   std::multimap<size_t, void*> pinned_free;         // pinned host blocks for reuse (batches)
@@ -1413,6 +1417,10 @@ extern "C" lj_status lj_batch_run(lj_batch* b, lj_star_result* results) {
 extern "C" void lj_batch_destroy(lj_batch* b) { batch_free(b); }
 
 // ------------------------------------------------------------ Bloom autotuner --
+// Filters up to this size (2 MiB) stay in an XCD's L2 next to the hash table's
+// hot lines; larger ones are mostly served from the MALL.
+constexpr int64_t kTuneL2Bits = 1ll << 24;
+
 // False-positive rate of a filter of m bits holding n keys.
 static double fpr_of(int hash, int layout, int k, double n, double m, double dim_rows) {
   if (n <= 0) return 0.0;
@@ -1424,17 +1432,25 @@ static double fpr_of(int hash, int layout, int k, double n, double m, double dim
     return 1.0 - std::pow(1.0 - sel, dim_rows / m);
   }
   if (layout == LJ_BLOOM_STANDARD) return std::pow(1.0 - std::exp(-(double)k * n / m), (double)k);
-  // blocked: k bits in one 32-bit word of m / 32; a word holds Poisson(lambda) keys
+  // blocked: k bits in one 32-bit word of m / 32; a word holds Poisson(lambda)
+  // keys.  The sum runs over lambda +- 12 standard deviations with each term
+  // in log space (exp(-lambda) alone underflows for a heavily loaded filter).
   const double words = m / 32.0, lambda = n / words;
-  double p = std::exp(-lambda), acc = 0.0;
-  for (int j = 0; j < 512; j++) {
-    if (j > 0) p *= lambda / j;
-    const double filled = 1.0 - std::pow(1.0 - 1.0 / 32.0, (double)k * j);
+  const double sd = std::sqrt(lambda), ll = std::log(lambda);
+  const int64_t j0 = (int64_t)std::max(0.0, std::floor(lambda - 12.0 * sd - 12.0));
+  const int64_t j1 = (int64_t)std::ceil(lambda + 12.0 * sd + 12.0);
+  double acc = 0.0;
+  for (int64_t j = j0; j <= j1; j++) {
+    const double p = std::exp(-lambda + (double)j * ll - std::lgamma((double)j + 1.0));
+    const double filled = 1.0 - std::pow(1.0 - 1.0 / 32.0, (double)k * (double)j);
     acc += p * std::pow(filled, (double)k);
-    if (j > lambda && p < 1e-12) break;
   }
-  return acc;
+  return std::min(1.0, acc);
 }
+
+// Fraction of 128-byte lines (32 rows) of a column that hold at least one
+// row passing with probability p (rows pass independently).
+static double lines_of(double p) { return 1.0 - std::pow(1.0 - p, 32.0); }
 
 extern "C" lj_status lj_tune_bloom(lj_ctx* c, int64_t n_selected, int64_t n_dim_rows, double fact_pass,
                                    lj_bloom_choice* out) {
@@ -1455,9 +1471,11 @@ extern "C" lj_status lj_tune_bloom(lj_ctx* c, int64_t n_selected, int64_t n_dim_
     for (const Cand& cd : cands) {
       const bool in_lds = (int64_t)m <= lds_bits;
       const double fpr = fpr_of(cd.hash, cd.layout, cd.k, (double)n_selected, m, (double)n_dim_rows);
-      double scan = in_lds ? 1.0 : c->tune_scan_gl;
-      if (cd.layout == LJ_BLOOM_STANDARD && cd.k > 1) scan += (cd.k - 1) * c->tune_probe_extra * (in_lds ? 1.0 : c->tune_scan_gl);
-      const double cost = scan + (fact_pass + (1.0 - fact_pass) * fpr) * c->tune_surv;
+      const double unit = in_lds ? 1.0 : m <= (double)kTuneL2Bits ? c->tune_scan_gl : c->tune_scan_big;
+      double scan = unit;
+      if (cd.layout == LJ_BLOOM_STANDARD && cd.k > 1) scan += (cd.k - 1) * c->tune_probe_extra * unit;
+      const double pass = fact_pass + (1.0 - fact_pass) * fpr;
+      const double cost = scan + lines_of(pass) * c->tune_line + pass * c->tune_gather;
       if (cost < best.cost * (1.0 - 1e-9)) {
         best.bloom_hash = cd.hash;
         best.bloom_layout = cd.layout;
@@ -1473,13 +1491,16 @@ extern "C" lj_status lj_tune_bloom(lj_ctx* c, int64_t n_selected, int64_t n_dim_
   return LJ_OK;
 }
 
-// Calibration: a dimension of 2^22 dense keys and a fact table of 2^24 rows
-// (device-generated), three LIP joins: a filter in LDS that nothing passes,
-// a filter in L2 that nothing passes, and an exact filter that 10 % pass.
+// Calibration: a dimension of 2^22 dense keys and a fact table of 2^25 rows
+// (device-generated), five LIP joins, each timed over all rows and over a
+// quarter of them (their difference is the cost per row): a filter in LDS
+// that nothing passes, a 2^22-bit (L2) and a 2^25-bit filter that nothing
+// passes, and the exact 2^22-bit filter that 1 % and that 40 % pass (their
+// survivor costs are measured against the empty filter of the same size).
 extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
   if (!c) return fail(c, LJ_EINVAL, "NULL context");
   HIP_TRY(c, hipSetDevice(c->device));
-  const int64_t nd = 1 << 22, nf = 1 << 24;
+  const int64_t nd = 1 << 22, nf = 1 << 25;
   int32_t *keys = nullptr, *attr = nullptr, *fk = nullptr, *meas = nullptr;
   lj_status s;
   if ((s = dev_alloc_t(c, nd, &keys)) != LJ_OK) return s;
@@ -1503,11 +1524,11 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
     release();
     return fail(c, LJ_EHIP, "generator launch failed");
   }
-  auto run = [&](int hash, int64_t bits, int32_t below, float* ms) -> lj_status {
+  auto run1 = [&](int hash, int64_t bits, int32_t below, int64_t n_fact, float* ms) -> lj_status {
     lj_star_desc q{};
     q.fact_fk[0] = fk;
     q.fact_measure = meas;
-    q.n_fact = nf;
+    q.n_fact = n_fact;
     q.n_dims = 1;
     q.method = LJ_METHOD_LIP;
     for (int i = 0; i < LJ_MAX_DIMS; i++) {
@@ -1538,17 +1559,33 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
     *ms = best;
     return LJ_OK;
   };
-  float t_lds = 0, t_gl = 0, t_pass = 0;
-  if ((s = run(LJ_BLOOM_MULT, 1 << 20, 0, &t_lds)) != LJ_OK || (s = run(LJ_BLOOM_MOD, 1 << 25, 0, &t_gl)) != LJ_OK ||
-      (s = run(LJ_BLOOM_MOD, nd, 1000, &t_pass)) != LJ_OK) {
+  // the cost per fact row: the difference between a run over all nf rows and
+  // one over a quarter of them (the launch, staging and finalize cancel out)
+  auto run = [&](int hash, int64_t bits, int32_t below, float* per_row) -> lj_status {
+    float t_all = 0, t_quarter = 0;
+    lj_status st = run1(hash, bits, below, nf, &t_all);
+    if (st == LJ_OK) st = run1(hash, bits, below, nf / 4, &t_quarter);
+    *per_row = (t_all - t_quarter) / (0.75f * (float)nf);
+    return st;
+  };
+  float t_lds = 0, t_l2 = 0, t_big = 0, t_p1 = 0, t_p40 = 0;
+  if ((s = run(LJ_BLOOM_MULT, 1 << 20, 0, &t_lds)) != LJ_OK || (s = run(LJ_BLOOM_MOD, nd, 0, &t_l2)) != LJ_OK ||
+      (s = run(LJ_BLOOM_MOD, 1 << 25, 0, &t_big)) != LJ_OK || (s = run(LJ_BLOOM_MOD, nd, 100, &t_p1)) != LJ_OK ||
+      (s = run(LJ_BLOOM_MOD, nd, 4000, &t_p40)) != LJ_OK) {
     release();
     return s;
   }
   release();
-  if (!(t_lds > 0 && t_gl > 0 && t_pass > 0)) return fail(c, LJ_EHIP, "calibration timings are not positive");
-  c->tune_scan_gl = std::max(1.0, (double)t_gl / t_lds);
-  // t_pass scans an L2 filter (2^22 bits = 512 KB) and passes 10 % of the rows
-  c->tune_surv = std::max(1.0, ((double)t_pass - t_gl) / 0.1 / t_lds);
+  if (!(t_lds > 0 && t_l2 > 0 && t_big > 0)) return fail(c, LJ_EHIP, "calibration timings are not positive");
+  c->tune_scan_gl = std::max(1.0, (double)t_l2 / t_lds);
+  c->tune_scan_big = std::max(c->tune_scan_gl, (double)t_big / t_lds);
+  // t_p1 / t_p40 scan the same 2^22-bit (512 KiB) filter as t_l2 and pass 1 % /
+  // 40 % of the rows: e(p) = lines_of(p) * line + p * gather, two equations
+  const double e1 = std::max(0.0, ((double)t_p1 - t_l2) / t_lds), e40 = std::max(0.0, ((double)t_p40 - t_l2) / t_lds);
+  const double a1 = lines_of(0.01), a40 = lines_of(0.4), det = a1 * 0.4 - a40 * 0.01;
+  const double line = (e1 * 0.4 - e40 * 0.01) / det, gather = (a1 * e40 - a40 * e1) / det;
+  c->tune_line = std::max(0.0, line);
+  c->tune_gather = std::max(1.0, gather);
   c->tune_calibrated = 1;
   return LJ_OK;
 }

@@ -313,9 +313,12 @@ lj_status lj_serve_stop(lj_server* srv);
  * Picks the Bloom filter (hash, layout, k, bits) of a LIP join on one
  * dimension: n_selected keys pass the dimension predicate, the fact side has
  * n_fact rows of which a fraction fact_pass joins.  The cost model per fact
- * row is scan(bits, k, in LDS or not) + P(pass) * survivor, with
- * P(pass) = fact_pass + (1 - fact_pass) * false-positive rate(bits, k, layout);
- * the constants are the context's (measured defaults, or lj_tune_calibrate). */
+ * row is scan(bits, k, where the filter lives: LDS, L2 up to 2 MiB, or
+ * beyond) + lines(P) * line + P * gather, with
+ * P = fact_pass + (1 - fact_pass) * false-positive rate(bits, k, layout) and
+ * lines(P) = 1 - (1 - P)^32 the fraction of 128-byte lines of the next column
+ * that hold a passing row; the constants are the context's (defaults, or
+ * measured by lj_tune_calibrate). */
 typedef struct {
   int32_t bloom_hash;      /* lj_bloom_hash */
   int32_t bloom_layout;    /* lj_bloom_layout */
