@@ -145,3 +145,10 @@ def test_tune_calibrate(ctx):
     ctx.tune_calibrate()
     ch = ctx.tune_bloom(200000, 1 << 22, 0.05)
     assert ch["bloom_bits"] >= 1 << 16 and 0.0 <= ch["fpr"] < 1.0
+
+
+def test_tune_bloom_loaded_filters(ctx):
+    # half of 2^22 keys selected: a filter with fewer bits than keys passes
+    # nearly every row, so it must not be chosen (its modelled FPR is ~1)
+    ch = ctx.tune_bloom(1 << 21, 1 << 22, 0.5)
+    assert ch["bloom_bits"] >= 1 << 21 and ch["fpr"] < 0.5
