@@ -182,6 +182,14 @@ def load(path=None):
     if not os.path.exists(p):
         raise LibraryMissing(
             f"{p} is missing: build it with `make -C crystal-opt_amd` (or __graft_entry__.build())")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7.  If
+    # torch is installed it is imported first, so that the library binds to
+    # torch's copy (same soname) instead of loading a second runtime, which
+    # then finds no device once torch has opened it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(p)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
