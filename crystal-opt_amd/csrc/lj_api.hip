@@ -798,6 +798,11 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
     for (int d = 1; d < nd; d++)
       if (kind_of(d) != cfg.k0) cfg.k0 = S0_GENERIC;
   }
+  // every filter of the kind (and placement) of stage 0: the kernel probes
+  // later stages with that kind's short test
+  a.all_k0 = lip && cfg.k0 != S0_GENERIC ? 1 : 0;
+  for (int d = 0; d < nd; d++)
+    if (lip && kind_of(d) != cfg.k0) a.all_k0 = 0;
   p.lds_bytes = probe_lds_bytes(cfg, lds_words);
   lj_status s = occupancy_of(c, cfg, p.lds_bytes, &p.per_cu);
   if (s != LJ_OK) return s;

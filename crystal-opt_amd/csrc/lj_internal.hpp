@@ -80,7 +80,7 @@ struct ProbeArgs {
   int32_t order[kMaxDims];        // hash-probe (join) order
   int32_t load_order[kMaxDims];   // column load order (NAIVE: order, LIP: Bloom order)
   int32_t lds_filter_words;
-  int32_t pad0;
+  int32_t all_k0;          // LIP: every dimension's filter has stage 0's kind and placement
   int64_t goffset;
   int64_t n_groups;
   unsigned long long* gsum;

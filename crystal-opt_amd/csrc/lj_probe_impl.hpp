@@ -82,14 +82,51 @@ __device__ __forceinline__ int lines_of_ballot(unsigned long long b) {
 }
 
 // ---- filter probes ----------------------------------------------------------
+typedef const __attribute__((address_space(3))) uint32_t* LdsWords;
+typedef const __attribute__((address_space(1))) uint32_t* GlobalWords;
+
+// A filter's words are read through a pointer typed with its address space
+// (LDS copy or global words, a uniform choice): a pointer that could be
+// either would compile to flat loads, each followed by a wait for all of the
+// wave's outstanding memory operations (the stream's prefetched tiles too).
 __device__ __forceinline__ bool bloom_probe_rt(const ProbeDim& D, const uint32_t* lds, int32_t key) {
   if (D.bh.kind == BLOOM_NONE) return true;
-  const uint32_t* bf = D.bloom_lds >= 0 ? lds + D.bloom_lds : D.bloom;
-  return bloom_test(D.bh, bf, key);
+  if (D.bloom_lds >= 0) return bloom_test(D.bh, (LdsWords)(lds + D.bloom_lds), key);
+  return bloom_test(D.bh, (GlobalWords)D.bloom, key);
 }
 
 __device__ __forceinline__ bool pht_probe(const ProbeDim& D, int32_t key, int32_t& pay) {
   return pht2_probe(D.ht, D.ht_len, D.keys_min, key, pay);
+}
+
+// A filter probe outside stage 0 (later stages, the adaptive sample).  When
+// every filter of the plan has stage 0's kind and placement (all_k0), the
+// probe is that kind's short test; otherwise the generic test.  Besides the
+// work, this keeps the executed code small: the generic test inlined at each
+// call site is thousands of instructions, and walking them misses the
+// instruction cache.  SPEC: 0 tests all_k0 here, 1 assumes it (the caller
+// tested it once for a whole loop), 2 assumes it is not set.
+template <int K0, int SPEC = 0>
+__device__ __forceinline__ bool probe_filter(const ProbeArgs& a, const ProbeDim& D, const uint32_t* lds,
+                                             int32_t key) {
+  const bool spec = SPEC == 1 || (SPEC == 0 && a.all_k0);
+  if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_GL) {
+    if (spec) {
+      const uint32_t x = (uint32_t)key - (uint32_t)D.bh.keys_min;
+      uint32_t h;
+      if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) h = x & D.bh.mod.pow2_mask;
+      else h = fastmod_u32(x, D.bh.mod);
+      uint32_t word;
+      if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS) word = ((LdsWords)(lds + D.bloom_lds))[h >> 5];
+      else word = ((GlobalWords)D.bloom)[h >> 5];
+      return key >= D.bh.keys_min && ((word >> (h & 31)) & 1u);
+    }
+  } else if constexpr (K0 == S0_MULT_LDS) {
+    if (spec) return bloom_test(D.bh, (LdsWords)(lds + D.bloom_lds), key);
+  } else if constexpr (K0 == S0_MULT_GL) {
+    if (spec) return bloom_test(D.bh, (GlobalWords)D.bloom, key);
+  }
+  return bloom_probe_rt(D, lds, key);
 }
 
 // Stage-0 parameters, hoisted out of the streaming loop.
@@ -339,8 +376,9 @@ struct WaveCtx {
 // stage 1's when bit 31 of row says the row passed stage 1 already (the
 // survivors of a dense segment).  With one dimension, bit 31 says that x1 is
 // the row's measure (read from the streamed measure column).
-template <bool LIP>
+template <int K0>
 __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int cnt) {
+  constexpr bool LIP = K0 != S0_PHT;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -379,9 +417,10 @@ __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int 
   w.p_k = k;
 }
 
-template <bool LIP>
+template <int K0>
 __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
                                                WaveStats& st) {
+  constexpr bool LIP = K0 != S0_PHT;
   if (w.p_cnt == 0) return;
   const int i = (int)lane_id();
   bool alive = i < w.p_cnt;
@@ -420,7 +459,7 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
         alive = pht_probe(D, k, p);
         pay.set(d, p);
       } else {
-        alive = bloom_probe_rt(D, lds, k);
+        alive = probe_filter<K0>(a, D, lds, k);
       }
     }
   }
@@ -471,12 +510,12 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
 
 // Finish the batch in flight and take the first kQueue queued rows as the
 // next one; the rest (fewer than kQueue) move to the front of the queue.
-template <bool LIP>
+template <int K0>
 __device__ __forceinline__ void rotate_batch(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
                                              WaveStats& st) {
-  complete_batch<LIP>(a, lds, gt, w, st);
+  complete_batch<K0>(a, lds, gt, w, st);
   const int take = w.qcount < kQueue ? w.qcount : kQueue;
-  issue_batch<LIP>(a, w, take);
+  issue_batch<K0>(a, w, take);
   const int rest = w.qcount - take;
   if (rest > 0) {
     // reads from [kQueue, kQueue + rest), writes to [0, rest): no overlap, and
@@ -501,11 +540,12 @@ __device__ __forceinline__ void rotate_batch(const ProbeArgs& a, const uint32_t*
 // round trip): returns true with the stage-1 survivors in qmask and their
 // stage-0 / stage-1 keys (LIP) or payloads (naive) in qx0 / qx1.  With one
 // dimension the segment is finished here (returns false).
-template <bool LIP>
+template <int K0>
 __device__ __forceinline__ bool process_dense(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
                                               uint32_t row0, const v4i32& key0, const int32_t (&pay0)[4],
                                               uint32_t mask, const v4i32& eager, bool use_eager, WaveStats& st,
                                               uint32_t& qmask, int32_t (&qx0)[4], int32_t (&qx1)[4]) {
+  constexpr bool LIP = K0 != S0_PHT;
   const int nd = w.nd;
   const int64_t n = a.n;
   Quad key[4], pay[4];
@@ -541,7 +581,7 @@ __device__ __forceinline__ bool process_dense(const ProbeArgs& a, const uint32_t
           ok = pht_probe(D, k, p);
           pay[j].set(d, p);
         } else {
-          ok = bloom_probe_rt(D, lds, k);
+          ok = probe_filter<K0>(a, D, lds, k);
         }
         nm |= (uint32_t)ok << j;
       }
@@ -745,7 +785,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
     total += cnt;
     if (cnt == 0) continue;
     // callers keep qcount < kQueue between tiles; later segments of a tile re-check
-    if (v > 0 && w.qcount >= kQueue) rotate_batch<LIP>(a, lds, gt, w, st);
+    if (v > 0 && w.qcount >= kQueue) rotate_batch<K0>(a, lds, gt, w, st);
     const uint32_t row0 = base + v * kSegRows + tc.lane4;
     int32_t qx0[4], qx1[4];
 #pragma unroll
@@ -763,7 +803,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
 #pragma unroll
         for (int j = 0; j < 4; j++) qx1[j] = vget(ev, j);
       }
-      if (w.nd >= 2 && !process_dense<LIP>(a, lds, gt, w, row0, kv, pv, m, ev, have_eager, st, qm, qx0, qx1))
+      if (w.nd >= 2 && !process_dense<K0>(a, lds, gt, w, row0, kv, pv, m, ev, have_eager, st, qm, qx0, qx1))
         continue;
       const int cq = popc_ballot4(qm);
       if (cq == 0) continue;
@@ -775,7 +815,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
         const uint32_t pm = parts == 1 ? qm : (((int)tc.lane4 >> 6) == part ? qm : 0u);
         const int c = parts == 1 ? cq : popc_ballot4(pm);
         if (c == 0) continue;
-        if (w.qcount + c > kQueueCap) rotate_batch<LIP>(a, lds, gt, w, st);
+        if (w.qcount + c > kQueueCap) rotate_batch<K0>(a, lds, gt, w, st);
         enqueue(w, row0, qx0, qx1, pm, d1, c);
       }
       continue;
@@ -793,7 +833,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
 // most selective first, and streams the next column with the first one when
 // the chosen first filter passes more than a quarter of the sampled rows.  The
 // tile itself is then processed like any other (its columns are in L2).
-template <int VEC>
+template <int VEC, int K0, int SPEC>
 __device__ __forceinline__ void sample_tile(const ProbeArgs& a, const uint32_t* lds_filter, WaveCtx& w,
                                             WaveStats& st, unsigned long long* lds_samp, uint32_t base,
                                             uint32_t lane4, int64_t n64, const v4i32 (&pre)[kMaxDims], bool& eager) {
@@ -814,7 +854,7 @@ __device__ __forceinline__ void sample_tile(const ProbeArgs& a, const uint32_t* 
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const bool vj = (valid >> j) & 1u;
-        m |= (uint32_t)(vj && bloom_probe_rt(a.dim[d], lds_filter, vget(kk, j))) << j;
+        m |= (uint32_t)(vj && probe_filter<K0, SPEC>(a, a.dim[d], lds_filter, vget(kk, j))) << j;
       }
       const uint32_t c = popc_ballot4(m);
       cnt[0] += d == 0 ? c : 0u;
@@ -928,7 +968,7 @@ __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, co
       // then vmcnt(#prefetch loads) instead of vmcnt(0))
       __builtin_amdgcn_sched_barrier(0);
       tile_step<VEC, K0, false>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
-      if (w.qcount >= kQueue) rotate_batch<K0 != S0_PHT>(a, lds, gt, w, st);
+      if (w.qcount >= kQueue) rotate_batch<K0>(a, lds, gt, w, st);
       if (tn >= n_full) break;
       t = tn;
       tn = sch.tile(raw);
@@ -949,7 +989,7 @@ __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, co
       raw2 = sch.issue();
       __builtin_amdgcn_sched_barrier(0);
       tile_step<VEC, K0, false>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
-      if (w.qcount >= kQueue) rotate_batch<K0 != S0_PHT>(a, lds, gt, w, st);
+      if (w.qcount >= kQueue) rotate_batch<K0>(a, lds, gt, w, st);
       if (tn >= n_full) break;
       t = tn;
       tn = tnn;
@@ -1069,8 +1109,12 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   const uint32_t n_full = n_rows / TILE;
   const uint32_t n_tiles = (n_rows + TILE - 1) / TILE;
   if constexpr (K0 != S0_PHT) {
-    if (a.adaptive && gw < n_tiles)
-      sample_tile<VEC>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, k_samp, eager);
+    // the filter-kind test is hoisted out of the sample's loops (one short,
+    // contiguous path when every filter has stage 0's kind)
+    if (a.adaptive && gw < n_tiles) {
+      if (a.all_k0) sample_tile<VEC, K0, 1>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, k_samp, eager);
+      else sample_tile<VEC, K0, 2>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, k_samp, eager);
+    }
   }
 
   // ---- streaming loop over tiles -------------------------------------------------
@@ -1120,7 +1164,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
         ke[v] = k[v];
       }
       const int tot = tile_step<VEC, K0, true>(a, s0, lds_filter, gt, w, st, tc, base, k, ke, false, oob);
-      if (w.qcount >= kQueue) rotate_batch<K0 != S0_PHT>(a, lds_filter, gt, w, st);
+      if (w.qcount >= kQueue) rotate_batch<K0>(a, lds_filter, gt, w, st);
       const uint32_t nv = tc.n - base < TILE ? tc.n - base : TILE;
       if (pass == 0 && !a.adaptive) eager = (uint32_t)tot * 4 > nv;
     }
@@ -1135,7 +1179,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   }
   // the batch in flight, then the rows still queued
 #pragma unroll 1
-  for (int it = 0; it < 2; it++) rotate_batch<K0 != S0_PHT>(a, lds_filter, gt, w, st);
+  for (int it = 0; it < 2; it++) rotate_batch<K0>(a, lds_filter, gt, w, st);
 
   // ---- flush: statistics and groups ----------------------------------------------
   {

@@ -17,7 +17,12 @@
 namespace lj {
 
 // True iff every bit of `key` is set in `bf` (global or LDS pointer).
-__device__ __forceinline__ bool bloom_test(const BloomHash& b, const uint32_t* bf, int32_t key) {
+// `Ptr` is a pointer to the filter's words: a plain pointer, or one qualified
+// with the LDS / global address space so that the reads compile to ds_read /
+// global_load rather than flat loads (a flat load waits on every outstanding
+// memory operation of the wave).
+template <typename Ptr>
+__device__ __forceinline__ bool bloom_test(const BloomHash& b, Ptr bf, int32_t key) {
   if (b.kind == BLOOM_MULT_BLOCKED) {
     uint32_t w, m;
     bloom_blocked_word(b, key, w, m);
