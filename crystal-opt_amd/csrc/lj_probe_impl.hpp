@@ -417,7 +417,7 @@ __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int 
   w.p_k = k;
 }
 
-template <int K0>
+template <int K0, int SPEC>
 __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
                                                WaveStats& st) {
   constexpr bool LIP = K0 != S0_PHT;
@@ -459,7 +459,7 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
         alive = pht_probe(D, k, p);
         pay.set(d, p);
       } else {
-        alive = probe_filter<K0>(a, D, lds, k);
+        alive = probe_filter<K0, SPEC>(a, D, lds, k);
       }
     }
   }
@@ -510,10 +510,10 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
 
 // Finish the batch in flight and take the first kQueue queued rows as the
 // next one; the rest (fewer than kQueue) move to the front of the queue.
-template <int K0>
+template <int K0, int SPEC>
 __device__ __forceinline__ void rotate_batch(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
                                              WaveStats& st) {
-  complete_batch<K0>(a, lds, gt, w, st);
+  complete_batch<K0, SPEC>(a, lds, gt, w, st);
   const int take = w.qcount < kQueue ? w.qcount : kQueue;
   issue_batch<K0>(a, w, take);
   const int rest = w.qcount - take;
@@ -540,7 +540,7 @@ __device__ __forceinline__ void rotate_batch(const ProbeArgs& a, const uint32_t*
 // round trip): returns true with the stage-1 survivors in qmask and their
 // stage-0 / stage-1 keys (LIP) or payloads (naive) in qx0 / qx1.  With one
 // dimension the segment is finished here (returns false).
-template <int K0>
+template <int K0, int SPEC>
 __device__ __forceinline__ bool process_dense(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
                                               uint32_t row0, const v4i32& key0, const int32_t (&pay0)[4],
                                               uint32_t mask, const v4i32& eager, bool use_eager, WaveStats& st,
@@ -581,7 +581,7 @@ __device__ __forceinline__ bool process_dense(const ProbeArgs& a, const uint32_t
           ok = pht_probe(D, k, p);
           pay[j].set(d, p);
         } else {
-          ok = probe_filter<K0>(a, D, lds, k);
+          ok = probe_filter<K0, SPEC>(a, D, lds, k);
         }
         nm |= (uint32_t)ok << j;
       }
@@ -738,7 +738,7 @@ struct TileCtx {
 // One tile: stage 0 on the registers `k` (VEC segments of 256 rows), then its
 // survivors.  CHECKED masks rows at or beyond n.  `ke` holds the next column
 // of the same rows when have_eager.  Returns the stage-0 survivor count.
-template <int VEC, int K0, bool CHECKED>
+template <int VEC, int K0, bool CHECKED, int SPEC>
 __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const uint32_t* lds, GroupLds& gt,
                                          WaveCtx& w, WaveStats& st, const TileCtx& tc, uint32_t base,
                                          const v4i32 (&k)[VEC], const v4i32 (&ke)[VEC], bool have_eager, int& oob) {
@@ -785,7 +785,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
     total += cnt;
     if (cnt == 0) continue;
     // callers keep qcount < kQueue between tiles; later segments of a tile re-check
-    if (v > 0 && w.qcount >= kQueue) rotate_batch<K0>(a, lds, gt, w, st);
+    if (v > 0 && w.qcount >= kQueue) rotate_batch<K0, SPEC>(a, lds, gt, w, st);
     const uint32_t row0 = base + v * kSegRows + tc.lane4;
     int32_t qx0[4], qx1[4];
 #pragma unroll
@@ -803,7 +803,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
 #pragma unroll
         for (int j = 0; j < 4; j++) qx1[j] = vget(ev, j);
       }
-      if (w.nd >= 2 && !process_dense<K0>(a, lds, gt, w, row0, kv, pv, m, ev, have_eager, st, qm, qx0, qx1))
+      if (w.nd >= 2 && !process_dense<K0, SPEC>(a, lds, gt, w, row0, kv, pv, m, ev, have_eager, st, qm, qx0, qx1))
         continue;
       const int cq = popc_ballot4(qm);
       if (cq == 0) continue;
@@ -815,7 +815,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
         const uint32_t pm = parts == 1 ? qm : (((int)tc.lane4 >> 6) == part ? qm : 0u);
         const int c = parts == 1 ? cq : popc_ballot4(pm);
         if (c == 0) continue;
-        if (w.qcount + c > kQueueCap) rotate_batch<K0>(a, lds, gt, w, st);
+        if (w.qcount + c > kQueueCap) rotate_batch<K0, SPEC>(a, lds, gt, w, st);
         enqueue(w, row0, qx0, qx1, pm, d1, c);
       }
       continue;
@@ -949,7 +949,7 @@ __device__ __forceinline__ void load_full_tile(const S0& s0, const TileCtx& tc, 
   }
 }
 
-template <int VEC, int K0, bool EAGER, int NW>
+template <int VEC, int K0, bool EAGER, int NW, int SPEC>
 __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, const uint32_t* lds, GroupLds& gt,
                                             WaveCtx& w, WaveStats& st, const TileCtx& tc, const TileSched<NW>& sch,
                                             uint32_t n_full, int& oob) {
@@ -967,8 +967,8 @@ __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, co
       // keep the prefetch ahead of the first use of `cur` (the wait below is
       // then vmcnt(#prefetch loads) instead of vmcnt(0))
       __builtin_amdgcn_sched_barrier(0);
-      tile_step<VEC, K0, false>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
-      if (w.qcount >= kQueue) rotate_batch<K0>(a, lds, gt, w, st);
+      tile_step<VEC, K0, false, SPEC>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
+      if (w.qcount >= kQueue) rotate_batch<K0, SPEC>(a, lds, gt, w, st);
       if (tn >= n_full) break;
       t = tn;
       tn = sch.tile(raw);
@@ -988,8 +988,8 @@ __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, co
       load_full_tile<VEC, EAGER>(s0, tc, tnn < n_full ? tnn : 0u, nx2, nx2e);
       raw2 = sch.issue();
       __builtin_amdgcn_sched_barrier(0);
-      tile_step<VEC, K0, false>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
-      if (w.qcount >= kQueue) rotate_batch<K0>(a, lds, gt, w, st);
+      tile_step<VEC, K0, false, SPEC>(a, s0, lds, gt, w, st, tc, tc.start + t * TILE, cur, cure, EAGER, oob);
+      if (w.qcount >= kQueue) rotate_batch<K0, SPEC>(a, lds, gt, w, st);
       if (tn >= n_full) break;
       t = tn;
       tn = tnn;
@@ -1006,7 +1006,7 @@ __device__ __forceinline__ void stream_loop(const ProbeArgs& a, const S0& s0, co
 
 // The probe of one query by workgroup `bid` of the `nblk` workgroups that
 // serve it (the whole grid for a single query; a slice of it in a batch).
-template <int WG, int VEC, int K0>
+template <int WG, int VEC, int K0, int SPEC>
 __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bid, const uint32_t nblk) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   constexpr int NW = WG / 64;
@@ -1109,12 +1109,8 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   const uint32_t n_full = n_rows / TILE;
   const uint32_t n_tiles = (n_rows + TILE - 1) / TILE;
   if constexpr (K0 != S0_PHT) {
-    // the filter-kind test is hoisted out of the sample's loops (one short,
-    // contiguous path when every filter has stage 0's kind)
-    if (a.adaptive && gw < n_tiles) {
-      if (a.all_k0) sample_tile<VEC, K0, 1>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, k_samp, eager);
-      else sample_tile<VEC, K0, 2>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, k_samp, eager);
-    }
+    if (a.adaptive && gw < n_tiles)
+      sample_tile<VEC, K0, SPEC>(a, lds_filter, w, st, lds_samp, gw * TILE, (uint32_t)lane * 4, n64, k_samp, eager);
   }
 
   // ---- streaming loop over tiles -------------------------------------------------
@@ -1163,8 +1159,8 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
         }
         ke[v] = k[v];
       }
-      const int tot = tile_step<VEC, K0, true>(a, s0, lds_filter, gt, w, st, tc, base, k, ke, false, oob);
-      if (w.qcount >= kQueue) rotate_batch<K0>(a, lds_filter, gt, w, st);
+      const int tot = tile_step<VEC, K0, true, SPEC>(a, s0, lds_filter, gt, w, st, tc, base, k, ke, false, oob);
+      if (w.qcount >= kQueue) rotate_batch<K0, SPEC>(a, lds_filter, gt, w, st);
       const uint32_t nv = tc.n - base < TILE ? tc.n - base : TILE;
       if (pass == 0 && !a.adaptive) eager = (uint32_t)tot * 4 > nv;
     }
@@ -1173,13 +1169,13 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       sch.next = lds_next;
       sch.stride = total_waves;
       sch.wg_base = bid * NW;
-      if (eager) stream_loop<VEC, K0, true, NW>(a, s0, lds_filter, gt, w, st, tc, sch, n_full, oob);
-      else stream_loop<VEC, K0, false, NW>(a, s0, lds_filter, gt, w, st, tc, sch, n_full, oob);
+      if (eager) stream_loop<VEC, K0, true, NW, SPEC>(a, s0, lds_filter, gt, w, st, tc, sch, n_full, oob);
+      else stream_loop<VEC, K0, false, NW, SPEC>(a, s0, lds_filter, gt, w, st, tc, sch, n_full, oob);
     }
   }
   // the batch in flight, then the rows still queued
 #pragma unroll 1
-  for (int it = 0; it < 2; it++) rotate_batch<K0>(a, lds_filter, gt, w, st);
+  for (int it = 0; it < 2; it++) rotate_batch<K0, SPEC>(a, lds_filter, gt, w, st);
 
   // ---- flush: statistics and groups ----------------------------------------------
   {
@@ -1227,6 +1223,22 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   }
 }
 
+// One copy of the probe per filter-kind case, chosen once per launch: with
+// every filter of stage 0's kind (all_k0) the later stages and the sample
+// run that kind's short test; otherwise the generic one.  Each copy is a
+// contiguous code path (the instruction cache sees only the one that runs).
+template <int WG, int VEC, int K0>
+__device__ __forceinline__ void probe_dispatch(const ProbeArgs& a, uint32_t bid, uint32_t nblk) {
+  constexpr bool has_spec = K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_GL ||
+                            K0 == S0_MULT_LDS || K0 == S0_MULT_GL;
+  if constexpr (has_spec) {
+    if (a.all_k0) probe_body<WG, VEC, K0, 1>(a, bid, nblk);
+    else probe_body<WG, VEC, K0, 2>(a, bid, nblk);
+  } else {
+    probe_body<WG, VEC, K0, 2>(a, bid, nblk);
+  }
+}
+
 template <int WG, int VEC, int K0>
 __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
   // Read the arguments in place from the kernarg segment: indexing the
@@ -1237,7 +1249,7 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
 #else
   const ProbeArgs& a = args;
 #endif
-  probe_body<WG, VEC, K0>(a, blockIdx.x, gridDim.x);
+  probe_dispatch<WG, VEC, K0>(a, blockIdx.x, gridDim.x);
 }
 
 // A batch of queries in one launch: query q is served by workgroups
@@ -1256,7 +1268,7 @@ __global__ __launch_bounds__(WG) void probe_multi_kernel(const ProbeArgs* __rest
     else hi = mid - 1;
   }
   const uint32_t b0 = wg_begin[lo];
-  probe_body<WG, VEC, K0>(qs[lo], blockIdx.x - b0, wg_begin[lo + 1] - b0);
+  probe_dispatch<WG, VEC, K0>(qs[lo], blockIdx.x - b0, wg_begin[lo + 1] - b0);
 }
 
 // Host-side table of instantiated kernels, filled by the instantiation units.
