@@ -867,17 +867,25 @@ __device__ __forceinline__ void sample_tile(const ProbeArgs& a, const uint32_t* 
     st.v[ST_SAMPLE_ROWS] += nv;
     st.v[ST_SAMPLE_LINES] += (long long)w.nd * lines_of_ballot(__ballot(valid != 0));
   }
-  // workgroup totals so far, including this wave's sample
-  uint32_t* samp = reinterpret_cast<uint32_t*>(lds_samp);
+  // workgroup totals so far, including this wave's sample: the pass counts
+  // of the 4 filters as 16-bit fields of one 64-bit LDS word (a workgroup
+  // samples fewer than 2^16 rows) and the rows sampled, two atomics a wave
   uint32_t tot[kMaxDims + 1] = {0, 0, 0, 0, 0};
-  if (lane_id() == 0) {
+  {
+    unsigned long long packed = 0, old = 0;
+    uint32_t rows_old = 0;
+#pragma unroll
+    for (int d = 0; d < kMaxDims; d++) packed |= (unsigned long long)(d < w.nd ? cnt[d] : 0u) << (16 * d);
+    if (lane_id() == 0) {
+      old = atomicAdd(&lds_samp[0], packed);
+      rows_old = atomicAdd(reinterpret_cast<uint32_t*>(&lds_samp[1]), n_valid);
+    }
+    const unsigned long long now = old + packed;
 #pragma unroll
     for (int d = 0; d < kMaxDims; d++)
-      if (d < w.nd) tot[d] = atomicAdd(&samp[d], cnt[d]) + cnt[d];
-    tot[kMaxDims] = atomicAdd(&samp[kMaxDims], n_valid) + n_valid;
+      tot[d] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((now >> (16 * d)) & 0xFFFFu));
+    tot[kMaxDims] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(rows_old + n_valid));
   }
-#pragma unroll
-  for (int d = 0; d <= kMaxDims; d++) tot[d] = (uint32_t)__builtin_amdgcn_readfirstlane((int)tot[d]);
   if (tot[kMaxDims] == 0) return;
   // rank of each dimension by pass count (ties: lower index first)
   uint32_t cntd[kMaxDims];
@@ -1010,6 +1018,7 @@ template <int WG, int VEC, int K0, int SPEC>
 __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bid, const uint32_t nblk) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   constexpr int NW = WG / 64;
+  static_assert(NW * VEC * kSegRows < (1 << 16), "adaptive sample totals are 16-bit fields");
   const int wave = uniform(threadIdx.x >> 6);  // known wave-uniform: keeps per-wave branches scalar
   const int lane = (int)lane_id();
 
