@@ -26,10 +26,11 @@
 //     processed in place with predicated 16-byte loads.  When a wave sees a
 //     high stage-0 pass rate it also prefetches the next column with the
 //     first one ("eager"), so dense tiles do not wait on a dependent load.
-//   * LIP_ADAPTIVE: every wave first probes ALL Bloom filters on one 256-row
-//     sample segment; the workgroup sums the pass counts in LDS and all its
-//     waves order the filters by ascending pass count.
-//   * Group-by: a 64-slot open-addressing table in LDS per workgroup holds
+//   * LIP_ADAPTIVE: every wave first probes ALL Bloom filters on every 4th
+//     row of one 256-row sample segment; the workgroup sums the pass counts
+//     in LDS and each wave orders the filters by ascending pass count (the
+//     totals so far).
+//   * Group-by: a 32-slot open-addressing table in LDS per workgroup holds
 //     int64 sums/counts, flushed with 64-bit global atomics at the end; a slot
 //     miss goes straight to the global atomics.  The first global touch of a
 //     group appends it to the `touched` list.
@@ -827,7 +828,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
 }
 
 // Adaptive LIP: a wave's first tile is its sample.  Every FK column of the
-// tile is loaded and every Bloom filter applied to every row; the per-filter
+// tile is loaded and every Bloom filter applied to every 4th row; the per-filter
 // pass counts go into the workgroup's LDS totals.  The wave then orders its
 // filters by the totals seen so far (its own sample at least; no barrier),
 // most selective first, and streams the next column with the first one when
@@ -850,20 +851,17 @@ __device__ __forceinline__ void sample_tile(const ProbeArgs& a, const uint32_t* 
     for (int d = 0; d < kMaxDims; d++) {
       if (d >= w.nd) continue;
       const v4i32 kk = v == 0 ? pre[d] : WaveLoadVec4<false>(a.dim[d].fk, row0, n64);
-      uint32_t m = 0;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const bool vj = (valid >> j) & 1u;
-        m |= (uint32_t)(vj && probe_filter<K0, SPEC>(a, a.dim[d], lds_filter, vget(kk, j))) << j;
-      }
-      const uint32_t c = popc_ballot4(m);
+      // the pass counts come from every 4th row (a lane's first row): the
+      // ranking needs rates, and a quarter of the probes keeps the sample short
+      const bool pass = (valid & 1u) && probe_filter<K0, SPEC>(a, a.dim[d], lds_filter, kk.x);
+      const uint32_t c = (uint32_t)__popcll(__ballot(pass));
       cnt[0] += d == 0 ? c : 0u;
       cnt[1] += d == 1 ? c : 0u;
       cnt[2] += d == 2 ? c : 0u;
       cnt[3] += d == 3 ? c : 0u;
     }
     const int nv = popc_ballot4(valid);
-    n_valid += nv;
+    n_valid += (uint32_t)__popcll(__ballot(valid & 1u));  // rows probed
     st.v[ST_SAMPLE_ROWS] += nv;
     st.v[ST_SAMPLE_LINES] += (long long)w.nd * lines_of_ballot(__ballot(valid != 0));
   }
