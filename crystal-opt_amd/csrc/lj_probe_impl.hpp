@@ -86,6 +86,15 @@ __device__ __forceinline__ int lines_of_ballot(unsigned long long b) {
 typedef const __attribute__((address_space(3))) uint32_t* LdsWords;
 typedef const __attribute__((address_space(1))) uint32_t* GlobalWords;
 
+// A column or table pointer of the query's arguments, typed as global memory
+// at the load.  The arguments of a batch are in device memory, and a pointer
+// loaded from there is "generic" to the compiler: a load through it would be
+// a flat load, which waits for both the vector-memory and the LDS counters.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gl(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+
 // A filter's words are read through a pointer typed with its address space
 // (LDS copy or global words, a uniform choice): a pointer that could be
 // either would compile to flat loads, each followed by a wait for all of the
@@ -96,8 +105,14 @@ __device__ __forceinline__ bool bloom_probe_rt(const ProbeDim& D, const uint32_t
   return bloom_test(D.bh, (GlobalWords)D.bloom, key);
 }
 
+// pht2_probe (join.hpp) with the slot read as a global load
 __device__ __forceinline__ bool pht_probe(const ProbeDim& D, int32_t key, int32_t& pay) {
-  return pht2_probe(D.ht, D.ht_len, D.keys_min, key, pay);
+  const int64_t s = pht_slot(key, D.ht_len, D.keys_min);
+  if (s < 0) return false;
+  const uint64_t slot = gl(reinterpret_cast<const uint64_t*>(D.ht))[s];
+  if (slot == 0) return false;
+  pay = (int32_t)(uint32_t)(slot >> 32);
+  return true;
 }
 
 // A filter probe outside stage 0 (later stages, the adaptive sample).  When
@@ -184,7 +199,7 @@ __device__ __forceinline__ uint32_t s0_probe4(const S0& s, const v4i32& k, uint3
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int64_t sl = pht_slot(kk[j], s.fm, s.kmin);
-      slot[j] = *reinterpret_cast<const uint64_t*>(s.ht + 2 * (sl < 0 ? 0 : sl));
+      slot[j] = gl(reinterpret_cast<const uint64_t*>(s.ht))[sl < 0 ? 0 : sl];
       ok |= (uint32_t)(sl >= 0) << j;
     }
 #pragma unroll
@@ -394,7 +409,7 @@ __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int 
   const int first = r < 0 ? 2 : 1;
   if (alive && first < w.nd) {
     const int32_t* col = first == 1 ? a.dim[w.ld1].fk : a.dim[w.ld2].fk;
-    k = col[(uint32_t)(r & 0x7FFFFFFF)];
+    k = gl(col)[(uint32_t)(r & 0x7FFFFFFF)];
   }
   // one dimension: the rows only need their hash slot (LIP) and measure; both
   // gathers go out now
@@ -402,11 +417,11 @@ __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int 
   int32_t m = 0;
   if (alive && w.nd == 1) {
     const uint32_t row = (uint32_t)(r & 0x7FFFFFFF);
-    m = r < 0 ? x1 : a.measure[row];
+    m = r < 0 ? x1 : gl(a.measure)[row];
     if (LIP) {
       const ProbeDim& D = a.dim[0];
       const int64_t sl = pht_slot(x0, D.ht_len, D.keys_min);
-      if (sl >= 0) ht = *reinterpret_cast<const uint64_t*>(D.ht + 2 * sl);
+      if (sl >= 0) ht = gl(reinterpret_cast<const uint64_t*>(D.ht + 2 * sl))[0];
     }
   }
   w.p_ht = ht;
@@ -451,7 +466,7 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
     st.v[ST_LINES_IN + s] += batch_lines(load, row, ls);
     w.set_last(s, ls);
     int32_t k = 0;
-    if (load) k = s == first ? w.p_k : D.fk[(uint32_t)row];
+    if (load) k = s == first ? w.p_k : gl(D.fk)[(uint32_t)row];
     st.v[ST_OOB] += __popcll(__ballot(load && k < D.keys_min));
     if (load) {
       key.set(d, k);
@@ -490,9 +505,9 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
         const ProbeDim& D = a.dim[on ? t : 0];
         const int64_t sl = pht_slot(key.get(t), D.ht_len, D.keys_min);
         hit = hit && (!on || sl >= 0);
-        slot[t] = *reinterpret_cast<const uint64_t*>(D.ht + 2 * (alive && on && sl >= 0 ? sl : 0));
+        slot[t] = gl(reinterpret_cast<const uint64_t*>(D.ht))[alive && on && sl >= 0 ? sl : 0];
       }
-      m = a.measure[alive ? (uint32_t)row : 0u];
+      m = gl(a.measure)[alive ? (uint32_t)row : 0u];
 #pragma unroll
       for (int t = 0; t < kMaxDims; t++) {
         if (t >= nd) continue;
@@ -566,7 +581,7 @@ __device__ __forceinline__ bool process_dense(const ProbeArgs& a, const uint32_t
     st.v[ST_ROWS_IN + s] += popc_ballot4(mask);
     v4i32 kv;
     if (s == 1 && use_eager) kv = eager;
-    else kv = WavePredLoadVec4(D.fk, row0, n, mask);
+    else kv = WavePredLoadVec4(gl(D.fk), row0, n, mask);
     uint32_t nm = 0, bad = 0;
 #pragma unroll
     for (int j = 0; j < 4; j++) bad |= (uint32_t)(((mask >> j) & 1u) && vget(kv, j) < D.keys_min) << j;
@@ -621,7 +636,7 @@ __device__ __forceinline__ bool process_dense(const ProbeArgs& a, const uint32_t
   st.meas_rows += popc_ballot4(mask);
   v4i32 mv;
   if (nd == 1 && use_eager) mv = eager;
-  else mv = WavePredLoadVec4(a.measure, row0, n, mask);
+  else mv = WavePredLoadVec4(gl(a.measure), row0, n, mask);
 #pragma unroll
   for (int j = 0; j < 4; j++) st.v[ST_OOB] += group_add(a, gt, (mask >> j) & 1u, pay[j], vget(mv, j));
   return false;
@@ -850,7 +865,7 @@ __device__ __forceinline__ void sample_tile(const ProbeArgs& a, const uint32_t* 
 #pragma unroll
     for (int d = 0; d < kMaxDims; d++) {
       if (d >= w.nd) continue;
-      const v4i32 kk = v == 0 ? pre[d] : WaveLoadVec4<false>(a.dim[d].fk, row0, n64);
+      const v4i32 kk = v == 0 ? pre[d] : WaveLoadVec4<false>(gl(a.dim[d].fk), row0, n64);
       // the pass counts come from every 4th row (a lane's first row): the
       // ranking needs rates, and a quarter of the probes keeps the sample short
       const bool pass = (valid & 1u) && probe_filter<K0, SPEC>(a, a.dim[d], lds_filter, kk.x);
@@ -949,9 +964,9 @@ __device__ __forceinline__ void load_full_tile(const S0& s0, const TileCtx& tc, 
 #pragma unroll
   for (int v = 0; v < VEC; v++) {
     const uint32_t r = tc.start + t * TILE + v * kSegRows + tc.lane4;
-    if (EAGER) e[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(tc.eager_col + r));
+    if (EAGER) e[v] = __builtin_nontemporal_load(gl(reinterpret_cast<const v4i32*>(tc.eager_col + r)));
     else e[v] = v4i32{0, 0, 0, 0};
-    k[v] = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(s0.fk + r));
+    k[v] = __builtin_nontemporal_load(gl(reinterpret_cast<const v4i32*>(s0.fk + r)));
   }
 }
 
@@ -1049,14 +1064,14 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
 #pragma unroll
   for (int v = 0; v < VEC; v++)
     k_first[v] = (int64_t)gw0 * TILE0 < n_first && !a.adaptive
-                     ? WaveLoadVec4<true>(a.dim[a.load_order[0]].fk, (int64_t)(gw0 * TILE0 + v * kSegRows + lane * 4), n_first)
+                     ? WaveLoadVec4<true>(gl(a.dim[a.load_order[0]].fk), (int64_t)(gw0 * TILE0 + v * kSegRows + lane * 4), n_first)
                      : v4i32{0, 0, 0, 0};
   // adaptive: every column of the sampled segment
   v4i32 k_samp[kMaxDims];
 #pragma unroll
   for (int d = 0; d < kMaxDims; d++)
     k_samp[d] = K0 != S0_PHT && a.adaptive && d < a.n_dims && (int64_t)gw0 * TILE0 < n_first
-                    ? WaveLoadVec4<false>(a.dim[d].fk, (int64_t)(gw0 * TILE0 + lane * 4), n_first)
+                    ? WaveLoadVec4<false>(gl(a.dim[d].fk), (int64_t)(gw0 * TILE0 + lane * 4), n_first)
                     : v4i32{0, 0, 0, 0};
 
   // ---- stage the LDS-resident filters, clear the group table
@@ -1162,7 +1177,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
           const v4i32 s0v = k_samp[0], s1v = k_samp[1], s2v = k_samp[2], s3v = k_samp[3];
           k[v] = l0 == 0 ? s0v : l0 == 1 ? s1v : l0 == 2 ? s2v : s3v;
         } else {
-          k[v] = WaveLoadVec4<true>(s0.fk, (int64_t)(base + v * kSegRows + tc.lane4), n64);
+          k[v] = WaveLoadVec4<true>(gl(s0.fk), (int64_t)(base + v * kSegRows + tc.lane4), n64);
         }
         ke[v] = k[v];
       }

@@ -78,13 +78,31 @@ __device__ __forceinline__ void BlockLoad(const T* inp, T (&items)[ITEMS_PER_THR
 
 namespace lj {
 
+// The 16-byte vector pointer of a column pointer, in the same address space.
+template <typename ColPtr>
+struct Vec4PtrOf;
+template <>
+struct Vec4PtrOf<const int32_t*> {
+  typedef const v4i32* type;
+};
+template <>
+struct Vec4PtrOf<int32_t*> {
+  typedef const v4i32* type;
+};
+template <>
+struct Vec4PtrOf<const __attribute__((address_space(1))) int32_t*> {
+  typedef const __attribute__((address_space(1))) v4i32* type;
+};
+
 // Rows [row0, row0+4) of `col` into v (row0 % 4 == 0, col 16-byte aligned).
-// Rows at or beyond n are not read and come back as 0.
-template <bool NONTEMPORAL>
-__device__ __forceinline__ v4i32 WaveLoadVec4(const int32_t* col, int64_t row0, int64_t n) {
+// Rows at or beyond n are not read and come back as 0.  `col` is a plain
+// column pointer or one typed with the global address space (the loads are
+// then global loads rather than flat ones).
+template <bool NONTEMPORAL, typename ColPtr>
+__device__ __forceinline__ v4i32 WaveLoadVec4(ColPtr col, int64_t row0, int64_t n) {
   v4i32 v;
   if (row0 + 4 <= n) {
-    const v4i32* p = reinterpret_cast<const v4i32*>(col + row0);
+    const typename Vec4PtrOf<ColPtr>::type p = reinterpret_cast<typename Vec4PtrOf<ColPtr>::type>(col + row0);
     if (NONTEMPORAL) v = __builtin_nontemporal_load(p);
     else v = *p;
   } else {
@@ -97,7 +115,8 @@ __device__ __forceinline__ v4i32 WaveLoadVec4(const int32_t* col, int64_t row0, 
 }
 
 // As WaveLoadVec4, but only when any of the 4 rows is alive (bit j of mask4).
-__device__ __forceinline__ v4i32 WavePredLoadVec4(const int32_t* col, int64_t row0, int64_t n, uint32_t mask4) {
+template <typename ColPtr>
+__device__ __forceinline__ v4i32 WavePredLoadVec4(ColPtr col, int64_t row0, int64_t n, uint32_t mask4) {
   v4i32 v = v4i32{0, 0, 0, 0};
   if (mask4) v = WaveLoadVec4<false>(col, row0, n);
   return v;
