@@ -241,6 +241,59 @@ def bench_serve(ctx, n_req, fractions=(0.25, 0.5, 0.75, 0.9, 1.1, 1.5, 2.0)):
     return out
 
 
+def bench_ssb(ctx, n, reps=3, sf=20):
+    """SSB flights 2 and 3 (src/ssb-jo/q{21..34}) at n fact rows over SF-sf
+    tables: each query in its six join orders x {naive, LIP}, and adaptive LIP.
+    Every plan's groups must agree; bytes_alg of a query is the smallest over
+    its LIP orders (128 B per touched line, the first column in full)."""
+    import torch
+    from lipjoin import ssb
+    from lipjoin import workloads as W
+    log(f"ssb flights 2/3: generating {n} lineorder rows at SF {sf}")
+    cols = ssb.device_columns(ctx, W.seed_of(7), n, sf)
+    torch.cuda.synchronize()
+    out = {}
+    for name in sorted(ssb.QUERIES):
+        q = ssb.QUERIES[name]
+        dims = ssb.dim_descs(q, cols, sf)
+        ref = None
+        per = {}
+        lip_bytes = []
+        for method in ("naive", "lip", "lip-adaptive"):
+            for oi, order in (ssb.ORDERS.items() if method != "lip-adaptive" else [(0, ssb.ORDERS[0])]):
+                desc = ssb.star(q, cols, n, method, order, sf, dims=dims)
+                res = ctx.star_join(desc, groups_cap=1 << 15)
+                ref = res["groups"] if ref is None else ref
+                assert res["groups"] == ref, f"{name} {method} order {oi} disagrees"
+                tq, tp = [], []
+                for _ in range(reps):
+                    r = ctx.star_join(desc, groups_cap=1 << 15)
+                    tq.append(r["time_query_ms"])
+                    tp.append(r["time_probe_ms"])
+                st = res["stats"]
+                b = 128 * (sum(st["lines_in"][:4]) + st["sample_lines"])
+                if method == "lip":
+                    lip_bytes.append(b)
+                key = method if method == "lip-adaptive" else f"{oi}-{method}"
+                per[key] = {"query_ms": statistics.median(tq), "probe_ms": statistics.median(tp)}
+        best_naive = min(v["query_ms"] for k, v in per.items() if k.endswith("-naive"))
+        ad = per["lip-adaptive"]
+        out[name] = {"groups": len(ref), "rows": sum(g[3] for g in ref),
+                     "adaptive_query_ms": ad["query_ms"], "adaptive_probe_ms": ad["probe_ms"],
+                     "adaptive_rows_per_s": n / (ad["query_ms"] * 1e-3),
+                     "bytes_alg": min(lip_bytes),
+                     "adaptive_probe_hbm_frac": min(lip_bytes) / (ad["probe_ms"] * 1e-3) / HBM_PEAK,
+                     "best_naive_query_ms": best_naive,
+                     "worst_naive_query_ms": max(v["query_ms"] for k, v in per.items() if k.endswith("-naive")),
+                     "best_lip_query_ms": min(v["query_ms"] for k, v in per.items() if k.endswith("-lip"))}
+        log(f"ssb {name}: adaptive {ad['query_ms']:.3f} ms (probe {ad['probe_ms']:.3f}, "
+            f"frac {out[name]['adaptive_probe_hbm_frac']:.2f}), best naive {best_naive:.3f} ms, "
+            f"{len(ref)} groups")
+    del cols
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(n_fact, seed, threads):
     """Oracle (C restatement of the reference semantics) on the host cores:
     the LIP plan in join order S->C->D, whole 120M-row input, median of 5
@@ -280,6 +333,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="config-5 requests per arrival rate (0 skips the serving measurement)")
     ap.add_argument("--no-single", action="store_true", help="skip the config 1/2 single-join measurements")
+    ap.add_argument("--no-ssb", action="store_true", help="skip the SSB flight 2/3 query measurements")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -404,10 +458,12 @@ def main():
         "lip_vs_naive_speedup": speedup,
         "plans": plans,
     }
+    # free the 120M-row columns first
+    del cols, descs
+    torch.cuda.empty_cache()
+    if not args.no_ssb:
+        out["ssb_flights"] = bench_ssb(ctx, n)
     if args.batch_instances > 0:
-        # free the 120M-row columns first
-        del cols, descs
-        torch.cuda.empty_cache()
         out["batched"] = bench_batch(ctx, args.batch_instances, reps=5)
     if args.serve_requests > 0:
         out["serving"] = bench_serve(ctx, args.serve_requests)

@@ -1,0 +1,197 @@
+"""The other SSB join-order queries (flights 2 and 3) as star-join descriptors.
+
+Each query of src/ssb-jo/q{21,22,23,31,32,33,34}/{0..5}-{naive,lip}.cu is one
+lj_star_desc: three dimensions, each with a predicate, a PHT_2 payload and a
+HASH1 Bloom filter of roundUpToPowerOfTwo(ht_len) bits, and a dense group
+index  g = (sum_d payload_d * group_mult[d] + group_offset) % n_groups.
+
+Synthetic SSB-shaped tables at scale factor SF (src/ssb/ssb_utils.h:13-19
+sizes: customer 30000*SF, supplier 2000*SF, part 200000*floor(1+log2 SF),
+date 2556 calendar days), with the dictionary encodings the reference's
+predicates assume:
+  nation ~ U[0,24], region = nation // 5, city = nation*10 + U[0,9]
+  category ~ U[0,24] (mfgr*5 + category), brand1 = category*40 + U[0,39]
+  d_year = calendar year, d_yearmonthnum = datekey // 100
+Lineorder: custkey ~ U[1,C], suppkey ~ U[1,S], partkey ~ U[1,P],
+orderdate uniform over the 2556 datekeys, revenue ~ U[1,10^6].
+
+Supplier's PHT_1 in Q2.x (q21/0-lip.cu:135, probed by BlockProbeAndPHT_1 at
+:77) is a PHT_2 here whose payload has group multiplier 0: with keys >= 1 both
+tables report a hit for exactly the inserted keys.  Flight 4 (q41-q43) sums
+lo_revenue - lo_supplycost, which the one-measure descriptor does not express.
+"""
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import (CMP_EQ, CMP_GTE, CMP_LTE, COMB_AND, COMB_OR, COMB_SET, GEN_DATEKEY, GEN_DATEKEY_SEQ,
+               GEN_SEQUENCE, GEN_UNIFORM, GEN_YEAR_OF_DAY, dim_desc, gen_host, round_up_pow2,
+               star_desc)
+
+DATE_MIN, DATE_MAX = 19920101, 19981230
+D_LEN = 2556
+D_VAL_LEN = DATE_MAX - DATE_MIN + 1
+
+
+def table_sizes(sf):
+    """(customer, supplier, part, date) rows at scale factor sf."""
+    return 30000 * sf, 2000 * sf, 200000 * int(math.floor(1 + math.log2(sf))), D_LEN
+
+
+def host_dims(seed, sf=1):
+    """customer/supplier/part/date columns (numpy int32)."""
+    c_len, s_len, p_len, d_len = table_sizes(sf)
+    cols = {}
+    for pre, n, st in (("c", c_len, 20), ("s", s_len, 23)):
+        nation = gen_host(GEN_UNIFORM, seed, st, 0, 24, n)
+        cols[f"{pre}_key"] = gen_host(GEN_SEQUENCE, seed, 0, 1, 0, n)
+        cols[f"{pre}_nation"] = nation
+        cols[f"{pre}_region"] = (nation // 5).astype(np.int32)
+        cols[f"{pre}_city"] = (nation * 10 + gen_host(GEN_UNIFORM, seed, st + 1, 0, 9, n)).astype(np.int32)
+    category = gen_host(GEN_UNIFORM, seed, 26, 0, 24, p_len)
+    cols["p_key"] = gen_host(GEN_SEQUENCE, seed, 0, 1, 0, p_len)
+    cols["p_category"] = category
+    cols["p_brand1"] = (category * 40 + gen_host(GEN_UNIFORM, seed, 27, 0, 39, p_len)).astype(np.int32)
+    cols["d_key"] = gen_host(GEN_DATEKEY_SEQ, seed, 0, 0, 0, d_len)
+    cols["d_year"] = gen_host(GEN_YEAR_OF_DAY, seed, 0, 0, 0, d_len)
+    cols["d_yearmonthnum"] = (cols["d_key"] // 100).astype(np.int32)
+    return cols
+
+
+def _fact_specs(sf):
+    """lineorder column -> (generator kind, stream, lo, hi)."""
+    c_len, s_len, p_len, d_len = table_sizes(sf)
+    return {
+        "lo_custkey": (GEN_UNIFORM, 30, 1, c_len),
+        "lo_suppkey": (GEN_UNIFORM, 31, 1, s_len),
+        "lo_partkey": (GEN_UNIFORM, 32, 1, p_len),
+        "lo_orderdate": (GEN_DATEKEY, 33, 0, d_len),
+        "lo_revenue": (GEN_UNIFORM, 34, 1, 1_000_000),
+    }
+
+
+def host_fact(seed, n, sf=1):
+    """lineorder columns (numpy int32) of n rows."""
+    return {k: gen_host(kind, seed, st, lo, hi, n) for k, (kind, st, lo, hi) in _fact_specs(sf).items()}
+
+
+def device_columns(ctx, seed, n, sf=1, torch_device="cuda"):
+    """Every table's columns as int32 device tensors: the dimensions generated on
+    the host and copied, lineorder generated on the device (the same bits as
+    host_fact)."""
+    import torch
+    cols = {k: torch.from_numpy(v).to(torch_device) for k, v in host_dims(seed, sf).items()}
+    for k, (kind, st, lo, hi) in _fact_specs(sf).items():
+        t = torch.empty(n, dtype=torch.int32, device=torch_device)
+        ctx.gen_device(t, kind, seed, st, lo, hi)
+        cols[k] = t
+    return cols
+
+
+# ------------------------------------------------------------------- queries --
+@dataclass(frozen=True)
+class DimQ:
+    """One dimension of a query: its table, predicate column and steps, payload."""
+    table: str          # "c", "s", "p" or "d"
+    fk: str             # lineorder column joined to the table's key
+    attr: str           # predicate column (None: every row selected)
+    payload: str        # PHT_2 value column
+    pred: tuple = ()    # ((comb, cmp, value), ...), crystal/pred.cuh semantics
+
+
+@dataclass(frozen=True)
+class Query:
+    name: str
+    dims: tuple                 # DimQ per descriptor dimension
+    group_mult: tuple
+    group_offset: int
+    n_groups: int
+    sum_bits: int               # width of the reference's atomicAdd: 32 (wraps) or 64
+    letters: str                # dimension letters in descriptor order, for join-order names
+    source: str                 # reference file:lines of the predicates and group index
+
+
+_YEARS_SET = ((COMB_SET, CMP_GTE, 1992), (COMB_SET, CMP_LTE, 1997))   # BlockPredGTE, BlockPredLTE
+_YEARS_AND = ((COMB_SET, CMP_GTE, 1992), (COMB_AND, CMP_LTE, 1997))   # BlockPredGTE, BlockPredAndLTE
+_CITIES = ((COMB_SET, CMP_EQ, 231), (COMB_OR, CMP_EQ, 235))
+
+
+def _flight2(name, s_region, part_pred, source):
+    # descriptor dims (S, P, D); g = (brand*7 + (year-1992)) % (7*1000)
+    return Query(name, (DimQ("s", "lo_suppkey", "s_region", "s_region", ((COMB_SET, CMP_EQ, s_region),)),
+                        DimQ("p", "lo_partkey", part_pred[0], "p_brand1", part_pred[1]),
+                        DimQ("d", "lo_orderdate", None, "d_year", ())),
+                 (0, 7, 1), -1992, 7 * 5 * 5 * 40, 64, "SPD", source)
+
+
+def _flight3(name, s_pred, c_pred, d_pred, payload, n_pay, sum_bits, source):
+    # descriptor dims (S, C, D); g = (s*n_pay*7 + c*7 + (year-1992)) % (7*n_pay*n_pay)
+    s_attr, s_steps = s_pred
+    c_attr, c_steps = c_pred
+    d_attr, d_steps = d_pred
+    return Query(name, (DimQ("s", "lo_suppkey", s_attr, f"s_{payload}", s_steps),
+                        DimQ("c", "lo_custkey", c_attr, f"c_{payload}", c_steps),
+                        DimQ("d", "lo_orderdate", d_attr, "d_year", d_steps)),
+                 (n_pay * 7, 7, 1), -1992, 7 * n_pay * n_pay, sum_bits, "SCD", source)
+
+
+QUERIES = {
+    "q21": _flight2("q21", 1, ("p_category", ((COMB_SET, CMP_EQ, 1),)),
+                    "q21/0-lip.cu:101-105,129,158,166,180-192"),
+    "q22": _flight2("q22", 2, ("p_brand1", ((COMB_SET, CMP_GTE, 260), (COMB_AND, CMP_LTE, 267))),
+                    "q22/0-lip.cu:101-105,129,157-160"),
+    "q23": _flight2("q23", 3, ("p_brand1", ((COMB_SET, CMP_EQ, 260),)),
+                    "q23/0-naive.cu:101-105,129,157"),
+    "q31": _flight3("q31", ("s_region", ((COMB_SET, CMP_EQ, 2),)), ("c_region", ((COMB_SET, CMP_EQ, 2),)),
+                    ("d_year", _YEARS_SET), "nation", 25, 64, "q31/0-lip.cu:102-109,135,166,196-199"),
+    "q32": _flight3("q32", ("s_nation", ((COMB_SET, CMP_EQ, 24),)), ("c_nation", ((COMB_SET, CMP_EQ, 24),)),
+                    ("d_year", _YEARS_AND), "city", 250, 32, "q32/0-lip.cu:102-108,133,164,194-197"),
+    "q33": _flight3("q33", ("s_city", _CITIES), ("c_city", _CITIES), ("d_year", _YEARS_SET), "city", 250, 32,
+                    "q33/0-lip.cu:98-111,132-135,192-195"),
+    "q34": _flight3("q34", ("s_city", _CITIES), ("c_city", _CITIES),
+                    ("d_yearmonthnum", ((COMB_SET, CMP_EQ, 199712),)), "city", 250, 32,
+                    "q34/0-lip.cu:102-108,132-135,194-195"),
+}
+
+
+# The six join orders of q{21..34}/{0..5}-*.cu (header line 1), as permutations of
+# the descriptor dimensions (X, Y, D): 0 X->Y->D, 1 X->D->Y, 2 D->X->Y, 3 D->Y->X,
+# 4 Y->D->X, 5 Y->X->D, with X = supplier and Y = part (flight 2) or customer (flight 3).
+ORDERS = {0: (0, 1, 2), 1: (0, 2, 1), 2: (2, 0, 1), 3: (2, 1, 0), 4: (1, 2, 0), 5: (1, 0, 2)}
+
+
+def _ht_len(table, sf):
+    c_len, s_len, p_len, _ = table_sizes(sf)
+    return {"c": c_len, "s": s_len, "p": p_len, "d": D_VAL_LEN}[table]
+
+
+def dim_params(dq, sf):
+    """(ht_len, keys_min, bloom_bits) of one dimension, as runQuery sizes them
+    (PHT_2 slots = table rows, the date table 19981230-19920101+1 slots with
+    keys_min 19920101; Bloom bits = roundUpToPowerOfTwo(slots), e.g.
+    q21/0-lip.cu:212-223)."""
+    n = _ht_len(dq.table, sf)
+    return n, (DATE_MIN if dq.table == "d" else 0), round_up_pow2(n)
+
+
+def dim_descs(q, cols, sf, bloom=True):
+    """lj_dim_desc per dimension from a dict of device columns (host_dims names)."""
+    out = []
+    for dq in q.dims:
+        ht_len, kmin, bits = dim_params(dq, sf)
+        n_rows = D_LEN if dq.table == "d" else _ht_len(dq.table, sf)
+        out.append(dim_desc(cols[f"{dq.table}_key"], n_rows, ht_len,
+                            attr=cols[dq.attr] if dq.attr else None, payload=cols[dq.payload],
+                            predicate=dq.pred, keys_min=kmin, bloom=("mod", bits) if bloom else None))
+    return out
+
+
+def star(q, cols, n_fact, method, order, sf, bloom_order=None, dims=None, resident=None):
+    """lj_star_desc of query q over device columns (dimension and lineorder);
+    resident: built dimensions (lj_dim_build) used instead of building in the call."""
+    if dims is None and resident is None:
+        dims = dim_descs(q, cols, sf)
+    return star_desc([cols[dq.fk] for dq in q.dims], cols["lo_revenue"], n_fact, dims, order,
+                     method=method, bloom_order=bloom_order, group_mult=q.group_mult,
+                     group_offset=q.group_offset, n_groups=q.n_groups, resident=resident)

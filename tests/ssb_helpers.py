@@ -1,0 +1,78 @@
+"""Oracle side of the SSB flight 2/3 parity tests, and an independent numpy
+evaluation of each query written from its SQL-level meaning (used to pin the
+oracle's predicate-step and group-index handling for these queries)."""
+import numpy as np
+
+import oracle as O
+from lipjoin import ssb
+
+SF = 1
+
+
+def oracle_dims(q, host_dims, sf=SF):
+    out = []
+    for dq in q.dims:
+        ht_len, kmin, bits = ssb.dim_params(dq, sf)
+        out.append(O.DimSpec(host_dims[f"{dq.table}_key"], host_dims[dq.attr] if dq.attr else None, ht_len,
+                             pred=list(dq.pred), payload=host_dims[dq.payload], keys_min=kmin,
+                             bloom=O.bloom_spec(O.BLOOM_MOD, bits, 1, keys_min=kmin)))
+    return out
+
+
+def oracle_query(q, host_dims, host_fact, order, method, sf=SF, threads=8):
+    """(groups, lines, rows, n_oob) from the oracle's star join."""
+    return O.star_join(oracle_dims(q, host_dims, sf), [host_fact[dq.fk] for dq in q.dims], host_fact["lo_revenue"],
+                       order, method=O.NAIVE if method == "naive" else O.LIP, group_mult=q.group_mult,
+                       group_offset=q.group_offset, n_groups=q.n_groups, threads=threads)
+
+
+def _row_of(table, key, host_dims):
+    if table == "d":
+        return np.searchsorted(host_dims["d_key"], key)
+    return key.astype(np.int64) - 1           # keys 1..n in row order
+
+
+def brute_force(name, host_dims, host_fact):
+    """[(g, payloads, sum, count)] ascending g, from the query's SQL meaning:
+    q21 s_region=1, p_category=1            group (d_year, p_brand1)
+    q22 s_region=2, p_brand1 in [260,267]   group (d_year, p_brand1)
+    q23 s_region=3, p_brand1=260            group (d_year, p_brand1)
+    q31 c_region=s_region=2, d_year<=1997   group (c_nation, s_nation, d_year)
+    q32 c_nation=s_nation=24, 1992<=d_year<=1997   group (c_city, s_city, d_year)
+    q33 c_city, s_city in {231,235}, d_year<=1997  group (c_city, s_city, d_year)
+    q34 c_city, s_city in {231,235}, d_yearmonthnum=199712
+    (d_year runs 1992..1998, so the reference's overwriting BlockPredLTE keeps
+    1992..1997 as the SQL does)."""
+    h, f = host_dims, host_fact
+    s = _row_of("s", f["lo_suppkey"], h)
+    d = _row_of("d", f["lo_orderdate"], h)
+    year = h["d_year"][d].astype(np.int64)
+    if name.startswith("q2"):
+        p = _row_of("p", f["lo_partkey"], h)
+        brand, region = h["p_brand1"][p].astype(np.int64), h["s_region"][s]
+        keep = {"q21": (region == 1) & (h["p_category"][p] == 1),
+                "q22": (region == 2) & (brand >= 260) & (brand <= 267),
+                "q23": (region == 3) & (brand == 260)}[name]
+        g = (brand * 7 + year - 1992) % 7000
+        pay = np.stack([region.astype(np.int64), brand, year], axis=1)
+    else:
+        c = _row_of("c", f["lo_custkey"], h)
+        col, npay = ("nation", 25) if name == "q31" else ("city", 250)
+        sv, cv = h[f"s_{col}"][s].astype(np.int64), h[f"c_{col}"][c].astype(np.int64)
+        if name == "q31":
+            keep = (h["s_region"][s] == 2) & (h["c_region"][c] == 2) & (year <= 1997)
+        elif name == "q32":
+            keep = (h["s_nation"][s] == 24) & (h["c_nation"][c] == 24) & (year >= 1992) & (year <= 1997)
+        else:
+            cities = lambda v: (v == 231) | (v == 235)
+            date_ok = (year <= 1997) if name == "q33" else (h["d_yearmonthnum"][d] == 199712)
+            keep = cities(sv) & cities(cv) & date_ok
+        g = (sv * npay * 7 + cv * 7 + year - 1992) % (7 * npay * npay)
+        pay = np.stack([sv, cv, year], axis=1)
+    rev = f["lo_revenue"].astype(np.int64)
+    out = {}
+    for gi, pi, ri in zip(g[keep], pay[keep], rev[keep]):
+        e = out.setdefault(int(gi), [tuple(int(x) for x in pi), 0, 0])
+        e[1] += int(ri)
+        e[2] += 1
+    return [(gi, e[0], e[1], e[2]) for gi, e in sorted(out.items())]

@@ -1,0 +1,39 @@
+"""The oracle's star join on the SSB flight 2/3 queries (predicate steps,
+payload columns and group encodings of src/ssb-jo/q{21..34}) against an
+independent numpy evaluation of each query's SQL meaning."""
+import pytest
+
+from lipjoin import ssb
+from lipjoin import workloads as W
+from ssb_helpers import SF, brute_force, oracle_query
+
+N_FACT = 1 << 18
+
+
+@pytest.fixture(scope="module")
+def data():
+    seed = W.seed_of(6)
+    return ssb.host_dims(seed, SF), ssb.host_fact(seed, N_FACT, SF)
+
+
+def test_dimension_encodings(data):
+    h, f = data
+    c_len, s_len, p_len, d_len = ssb.table_sizes(SF)
+    assert h["c_key"].size == c_len and h["s_key"].size == s_len and h["p_key"].size == p_len
+    assert (h["s_region"] == h["s_nation"] // 5).all() and (h["c_city"] // 10 == h["c_nation"]).all()
+    assert (h["p_brand1"] // 40 == h["p_category"]).all() and h["p_brand1"].max() < 1000
+    assert set(h["d_year"].tolist()) == set(range(1992, 1999))
+    assert h["d_yearmonthnum"].min() == 199201 and h["d_yearmonthnum"].max() == 199812
+    assert f["lo_partkey"].min() >= 1 and f["lo_partkey"].max() <= p_len
+
+
+@pytest.mark.parametrize("name", sorted(ssb.QUERIES))
+@pytest.mark.parametrize("method,order", [("naive", 0), ("lip", 0), ("naive", 3), ("lip", 4)])
+def test_oracle_matches_sql_meaning(data, name, method, order):
+    h, f = data
+    q = ssb.QUERIES[name]
+    groups, _, rows, oob = oracle_query(q, h, f, ssb.ORDERS[order], method)
+    want = brute_force(name, h, f)
+    assert oob == 0
+    assert len(want) > 0, "the query selects no row at this size"
+    assert groups == want
