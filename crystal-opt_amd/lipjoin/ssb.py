@@ -10,21 +10,27 @@ sizes: customer 30000*SF, supplier 2000*SF, part 200000*floor(1+log2 SF),
 date 2556 calendar days), with the dictionary encodings the reference's
 predicates assume:
   nation ~ U[0,24], region = nation // 5, city = nation*10 + U[0,9]
-  category ~ U[0,24] (mfgr*5 + category), brand1 = category*40 + U[0,39]
+  category ~ U[0,24] (mfgr*5 + category), mfgr = category // 5,
+  brand1 = category*40 + U[0,39]
   d_year = calendar year, d_yearmonthnum = datekey // 100
 Lineorder: custkey ~ U[1,C], suppkey ~ U[1,S], partkey ~ U[1,P],
-orderdate uniform over the 2556 datekeys, revenue ~ U[1,10^6].
+orderdate uniform over the 2556 datekeys, revenue ~ U[1,10^6],
+supplycost ~ U[1,10^5].
 
 Supplier's PHT_1 in Q2.x (q21/0-lip.cu:135, probed by BlockProbeAndPHT_1 at
 :77) is a PHT_2 here whose payload has group multiplier 0: with keys >= 1 both
-tables report a hit for exactly the inserted keys.  Flight 4 (q41-q43) sums
-lo_revenue - lo_supplycost, which the one-measure descriptor does not express.
+tables report a hit for exactly the inserted keys.  Q4.1 (q41/0-lip.cu) sums
+lo_revenue - lo_supplycost (:127-128): it runs as two probes of the same plan,
+one per measure column, and the per-group sums are subtracted.  Both probes
+select the same rows into the same groups, so the int64 difference equals the
+reference's int64 sum of the int32 differences exactly.
 """
 import math
 from dataclasses import dataclass
 
 import numpy as np
 
+from . import _ptr
 from . import (CMP_EQ, CMP_GTE, CMP_LTE, COMB_AND, COMB_OR, COMB_SET, GEN_DATEKEY, GEN_DATEKEY_SEQ,
                GEN_SEQUENCE, GEN_UNIFORM, GEN_YEAR_OF_DAY, dim_desc, gen_host, round_up_pow2,
                star_desc)
@@ -52,6 +58,7 @@ def host_dims(seed, sf=1):
     category = gen_host(GEN_UNIFORM, seed, 26, 0, 24, p_len)
     cols["p_key"] = gen_host(GEN_SEQUENCE, seed, 0, 1, 0, p_len)
     cols["p_category"] = category
+    cols["p_mfgr"] = (category // 5).astype(np.int32)
     cols["p_brand1"] = (category * 40 + gen_host(GEN_UNIFORM, seed, 27, 0, 39, p_len)).astype(np.int32)
     cols["d_key"] = gen_host(GEN_DATEKEY_SEQ, seed, 0, 0, 0, d_len)
     cols["d_year"] = gen_host(GEN_YEAR_OF_DAY, seed, 0, 0, 0, d_len)
@@ -68,6 +75,7 @@ def _fact_specs(sf):
         "lo_partkey": (GEN_UNIFORM, 32, 1, p_len),
         "lo_orderdate": (GEN_DATEKEY, 33, 0, d_len),
         "lo_revenue": (GEN_UNIFORM, 34, 1, 1_000_000),
+        "lo_supplycost": (GEN_UNIFORM, 35, 1, 100_000),
     }
 
 
@@ -110,6 +118,7 @@ class Query:
     sum_bits: int               # width of the reference's atomicAdd: 32 (wraps) or 64
     letters: str                # dimension letters in descriptor order, for join-order names
     source: str                 # reference file:lines of the predicates and group index
+    measure_sub: str = None     # lineorder column subtracted from lo_revenue (Q4.x)
 
 
 _YEARS_SET = ((COMB_SET, CMP_GTE, 1992), (COMB_SET, CMP_LTE, 1997))   # BlockPredGTE, BlockPredLTE
@@ -149,6 +158,15 @@ QUERIES = {
                     ("d_year", _YEARS_AND), "city", 250, 32, "q32/0-lip.cu:102-108,133,164,194-197"),
     "q33": _flight3("q33", ("s_city", _CITIES), ("c_city", _CITIES), ("d_year", _YEARS_SET), "city", 250, 32,
                     "q33/0-lip.cu:98-111,132-135,192-195"),
+    # descriptor dims (S, C, P, D), the order of q41/0-{naive,lip}.cu;
+    # g = (c_nation*7 + (year-1992)) % (7*25)
+    "q41": Query("q41", (DimQ("s", "lo_suppkey", "s_region", "s_region", ((COMB_SET, CMP_EQ, 1),)),
+                         DimQ("c", "lo_custkey", "c_region", "c_nation", ((COMB_SET, CMP_EQ, 1),)),
+                         DimQ("p", "lo_partkey", "p_mfgr", "p_mfgr",
+                              ((COMB_SET, CMP_EQ, 0), (COMB_OR, CMP_EQ, 1))),
+                         DimQ("d", "lo_orderdate", None, "d_year", ())),
+                 (0, 7, 0, 1), -1992, 7 * 25, 64, "SCPD",
+                 "q41/0-lip.cu:120-128,151,157,178-186,209,217,237-243", measure_sub="lo_supplycost"),
     "q34": _flight3("q34", ("s_city", _CITIES), ("c_city", _CITIES),
                     ("d_yearmonthnum", ((COMB_SET, CMP_EQ, 199712),)), "city", 250, 32,
                     "q34/0-lip.cu:102-108,132-135,194-195"),
@@ -159,6 +177,12 @@ QUERIES = {
 # the descriptor dimensions (X, Y, D): 0 X->Y->D, 1 X->D->Y, 2 D->X->Y, 3 D->Y->X,
 # 4 Y->D->X, 5 Y->X->D, with X = supplier and Y = part (flight 2) or customer (flight 3).
 ORDERS = {0: (0, 1, 2), 1: (0, 2, 1), 2: (2, 0, 1), 3: (2, 1, 0), 4: (1, 2, 0), 5: (1, 0, 2)}
+# q41 ships one order, S->C->P->D (q41/0-{naive,lip}.cu line 1)
+ORDERS_4 = {0: (0, 1, 2, 3)}
+
+
+def orders_of(q):
+    return ORDERS if len(q.dims) == 3 else ORDERS_4
 
 
 def _ht_len(table, sf):
@@ -195,3 +219,27 @@ def star(q, cols, n_fact, method, order, sf, bloom_order=None, dims=None, reside
     return star_desc([cols[dq.fk] for dq in q.dims], cols["lo_revenue"], n_fact, dims, order,
                      method=method, bloom_order=bloom_order, group_mult=q.group_mult,
                      group_offset=q.group_offset, n_groups=q.n_groups, resident=resident)
+
+
+def run(ctx, q, cols, n_fact, method, order, sf, dims=None, resident=None, groups_cap=1 << 14):
+    """lj_star_join of query q (Context.star_join's result dict).  A query with
+    a subtracted measure runs the plan twice, on lo_revenue and on the
+    subtracted column, and returns the per-group differences (times summed)."""
+    desc = star(q, cols, n_fact, method, order, sf, dims=dims, resident=resident)
+    res = ctx.star_join(desc, groups_cap=groups_cap)
+    if q.measure_sub is None:
+        return res
+    desc.fact_measure = _ptr(cols[q.measure_sub])
+    sub = ctx.star_join(desc, groups_cap=groups_cap)
+    if [(g[0], g[3]) for g in sub["groups"]] != [(g[0], g[3]) for g in res["groups"]]:
+        raise RuntimeError("the two measure passes selected different groups")
+    groups = []
+    for a, b in zip(res["groups"], sub["groups"]):
+        d = a[2] - b[2]
+        groups.append((a[0], a[1], d, a[3], (d + 2**31) % 2**32 - 2**31))
+    out = dict(res)
+    out["groups"] = groups
+    out["sum_total"] = res["sum_total"] - sub["sum_total"]
+    for k in ("time_query_ms", "time_build_ms", "time_probe_ms"):
+        out[k] = res[k] + sub[k]
+    return out

@@ -20,10 +20,25 @@ def oracle_dims(q, host_dims, sf=SF):
 
 
 def oracle_query(q, host_dims, host_fact, order, method, sf=SF, threads=8):
-    """(groups, lines, rows, n_oob) from the oracle's star join."""
-    return O.star_join(oracle_dims(q, host_dims, sf), [host_fact[dq.fk] for dq in q.dims], host_fact["lo_revenue"],
-                       order, method=O.NAIVE if method == "naive" else O.LIP, group_mult=q.group_mult,
-                       group_offset=q.group_offset, n_groups=q.n_groups, threads=threads)
+    """(groups, lines, rows, n_oob) from the oracle's star join; a query with a
+    subtracted measure runs twice and returns the per-group differences."""
+    def one(measure):
+        return O.star_join(oracle_dims(q, host_dims, sf), [host_fact[dq.fk] for dq in q.dims], measure,
+                           order, method=O.NAIVE if method == "naive" else O.LIP, group_mult=q.group_mult,
+                           group_offset=q.group_offset, n_groups=q.n_groups, threads=threads)
+    groups, lines, rows, oob = one(host_fact["lo_revenue"])
+    if q.measure_sub is not None:
+        sub = one(host_fact[q.measure_sub])[0]
+        assert [(a[0], a[3]) for a in groups] == [(b[0], b[3]) for b in sub]
+        groups = [(a[0], a[1], a[2] - b[2], a[3]) for a, b in zip(groups, sub)]
+    return groups, lines, rows, oob
+
+
+def canon(q, groups):
+    """Groups with the payloads of dimensions outside the group key (multiplier
+    0) zeroed: such a payload is any selected row's value, not a result."""
+    keep = [m != 0 for m in q.group_mult]
+    return [(g[0], tuple(p if k else 0 for p, k in zip(g[1], keep))) + tuple(g[2:]) for g in groups]
 
 
 def _row_of(table, key, host_dims):
@@ -41,12 +56,24 @@ def brute_force(name, host_dims, host_fact):
     q32 c_nation=s_nation=24, 1992<=d_year<=1997   group (c_city, s_city, d_year)
     q33 c_city, s_city in {231,235}, d_year<=1997  group (c_city, s_city, d_year)
     q34 c_city, s_city in {231,235}, d_yearmonthnum=199712
+    q41 c_region=s_region=1, p_mfgr in {0,1}, SUM(revenue - supplycost)
+                                            group (d_year, c_nation)
     (d_year runs 1992..1998, so the reference's overwriting BlockPredLTE keeps
     1992..1997 as the SQL does)."""
     h, f = host_dims, host_fact
     s = _row_of("s", f["lo_suppkey"], h)
     d = _row_of("d", f["lo_orderdate"], h)
     year = h["d_year"][d].astype(np.int64)
+    if name == "q41":
+        c = _row_of("c", f["lo_custkey"], h)
+        p = _row_of("p", f["lo_partkey"], h)
+        mfgr, nation = h["p_mfgr"][p].astype(np.int64), h["c_nation"][c].astype(np.int64)
+        region = h["s_region"][s].astype(np.int64)
+        keep = (region == 1) & (h["c_region"][c] == 1) & ((mfgr == 0) | (mfgr == 1))
+        g = (nation * 7 + year - 1992) % 175
+        pay = np.stack([region, nation, mfgr, year], axis=1)
+        measure = f["lo_revenue"].astype(np.int64) - f["lo_supplycost"].astype(np.int64)
+        return _aggregate(g, pay, measure, keep)
     if name.startswith("q2"):
         p = _row_of("p", f["lo_partkey"], h)
         brand, region = h["p_brand1"][p].astype(np.int64), h["s_region"][s]
@@ -69,7 +96,10 @@ def brute_force(name, host_dims, host_fact):
             keep = cities(sv) & cities(cv) & date_ok
         g = (sv * npay * 7 + cv * 7 + year - 1992) % (7 * npay * npay)
         pay = np.stack([sv, cv, year], axis=1)
-    rev = f["lo_revenue"].astype(np.int64)
+    return _aggregate(g, pay, f["lo_revenue"].astype(np.int64), keep)
+
+
+def _aggregate(g, pay, rev, keep):
     out = {}
     for gi, pi, ri in zip(g[keep], pay[keep], rev[keep]):
         e = out.setdefault(int(gi), [tuple(int(x) for x in pi), 0, 0])
