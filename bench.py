@@ -250,8 +250,12 @@ def bench_ssb(ctx, n, reps=3, sf=20):
     in full; Q4.1's two measure passes are both counted by their timings, its
     bytes_alg is one pass)."""
     import torch
+    import lipjoin as L
     from lipjoin import ssb
     from lipjoin import workloads as W
+    # a context of its own: group accumulators sized for these group spaces
+    # (the Q3.3 space of the headline context is 437,500 groups)
+    ctx = L.Context(0)
     log(f"ssb flights 2-4: generating {n} lineorder rows at SF {sf}")
     cols = ssb.device_columns(ctx, W.seed_of(7), n, sf)
     torch.cuda.synchronize()
@@ -296,6 +300,7 @@ def bench_ssb(ctx, n, reps=3, sf=20):
             f"frac {out[name]['adaptive_probe_hbm_frac']:.2f}), best naive {best_naive:.3f} ms, "
             f"{len(ref)} groups")
     del cols
+    ctx.close()
     torch.cuda.empty_cache()
     return out
 

@@ -75,6 +75,8 @@ struct lj_ctx {
   unsigned long long* gcount = nullptr;
   unsigned long long* gprod = nullptr;
   int32_t* gpay = nullptr;
+  uint32_t* gseen = nullptr;       // [g_cap] when g_reps > 1
+  int32_t g_reps = 1;              // copies of gsum/gcount/gprod, g_cap apart
   uint32_t* touched = nullptr;
   uint32_t touched_cap = 0;
   uint32_t* n_touched = nullptr;  // [0] groups hit; [32] workgroups finished (probe kernel)
@@ -613,15 +615,21 @@ static lj_status ensure_groups(lj_ctx* c, int64_t n_groups) {
   dev_release(c, c->gcount);
   dev_release(c, c->gprod);
   dev_release(c, c->gpay);
+  dev_release(c, c->gseen);
   dev_release(c, c->touched);
   dev_release(c, c->out_buf);
   c->out_buf = nullptr;
+  c->gseen = nullptr;
   c->g_cap = 0;
+  c->g_reps = 1;
   const int64_t cap = n_groups;
+  const int reps = cap <= kRepGroups ? kGroupReps : 1;
   const uint32_t tcap = (uint32_t)std::min<int64_t>(n_groups, 1 << 20);
   lj_status s;
-  if ((s = dev_alloc_t(c, cap, &c->gsum)) != LJ_OK || (s = dev_alloc_t(c, cap, &c->gcount)) != LJ_OK ||
-      (s = dev_alloc_t(c, cap, &c->gprod)) != LJ_OK || (s = dev_alloc_t(c, cap * kMaxDims, &c->gpay)) != LJ_OK ||
+  if ((s = dev_alloc_t(c, cap * reps, &c->gsum)) != LJ_OK || (s = dev_alloc_t(c, cap * reps, &c->gcount)) != LJ_OK ||
+      (s = dev_alloc_t(c, cap * reps, &c->gprod)) != LJ_OK ||
+      (s = dev_alloc_t(c, cap * kMaxDims, &c->gpay)) != LJ_OK ||
+      (reps > 1 && (s = dev_alloc_t(c, cap, &c->gseen)) != LJ_OK) ||
       (s = dev_alloc_t(c, (int64_t)tcap, &c->touched)) != LJ_OK ||
       (s = dev_alloc(c, kHdrBytes + sizeof(GroupOut) * tcap, &c->out_buf)) != LJ_OK)
     return s;
@@ -629,12 +637,14 @@ static lj_status ensure_groups(lj_ctx* c, int64_t n_groups) {
   c->out = reinterpret_cast<GroupOut*>(static_cast<char*>(c->out_buf) + kHdrBytes);
   if (!c->n_touched && (s = dev_alloc_t(c, 64, &c->n_touched)) != LJ_OK) return s;  // [0] touched, [32] done
   FillArgs f{};
-  add_fill(f, c->gsum, cap * 2);
-  add_fill(f, c->gcount, cap * 2);
-  add_fill(f, c->gprod, cap * 2);
+  add_fill(f, c->gsum, cap * reps * 2);
+  add_fill(f, c->gcount, cap * reps * 2);
+  add_fill(f, c->gprod, cap * reps * 2);
+  if (reps > 1) add_fill(f, c->gseen, cap);
   add_fill(f, c->n_touched, 64);
   HIP_TRY(c, launch_fill(f, c->stream));
   c->g_cap = cap;
+  c->g_reps = reps;
   c->touched_cap = tcap;
   return LJ_OK;
 }
@@ -894,6 +904,12 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   a.gcount = c->gcount;
   a.gprod = c->gprod;
   a.gpay = c->gpay;
+  if (c->g_reps > 1) {
+    // the allocation holds g_reps copies of g_cap >= n_groups entries
+    a.g_reps = c->g_reps;
+    a.g_stride = c->g_cap;
+    a.gseen = c->gseen;
+  }
   a.touched = c->touched;
   a.n_touched = c->n_touched;
   a.touched_cap = c->touched_cap;

@@ -12,6 +12,12 @@ namespace lj {
 
 constexpr int kMaxDims = 4;
 constexpr int kGroupSlots = 32;      // LDS-privatised groups per workgroup
+// Group spaces up to kRepGroups keep kGroupReps copies of the global group
+// accumulators (copy = workgroup index mod 8, i.e. one per XCD under the
+// round-robin dispatch), so that groups overflowing the LDS table are not all
+// added into the same few cache lines by every CU.
+constexpr int kGroupReps = 8;
+constexpr int64_t kRepGroups = 1 << 16;
 constexpr int kQueue = 64;           // survivor rows per batch (one per lane)
 constexpr int kQueueCap = kQueue + 8; // survivor queue entries per wave
 constexpr int kStats = 16;           // per-workgroup statistics words
@@ -99,6 +105,13 @@ struct ProbeArgs {
   GroupOut* out;
   int64_t out_cap;
   const unsigned long long* dim_counters[kMaxDims];
+  // replicated accumulators (single queries with n_groups <= kRepGroups):
+  // gsum/gcount/gprod hold g_reps copies g_stride apart; gseen[g] marks a
+  // group as listed in `touched`.  g_reps 0 or 1: one copy, gseen unused.
+  uint32_t* gseen;
+  int64_t g_stride;
+  int32_t g_reps;
+  int32_t pad2;
   // if set (a single query): the header and the first host_groups groups are
   // also written to this pinned host memory, so no copy follows the probe
   long long* host_hdr;

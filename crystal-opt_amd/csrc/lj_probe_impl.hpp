@@ -264,6 +264,21 @@ struct GroupLds {
 
 __device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, long long s, unsigned long long c,
                                                  unsigned long long p, const Quad& pay, int nd) {
+  if (a.g_reps > 1) {
+    // this workgroup's copy; the first add of a group to any copy lists it once
+    const int64_t gi = g + (int64_t)(blockIdx.x & (uint32_t)(a.g_reps - 1)) * a.g_stride;
+    atomicAdd(&a.gsum[gi], (unsigned long long)s);
+    if (a.want_product) atomicAdd(&a.gprod[gi], p);
+    const unsigned long long old = atomicAdd(&a.gcount[gi], c);
+    if (old == 0 && atomicExch(&a.gseen[g], 1u) == 0u) {
+#pragma unroll
+      for (int d = 0; d < kMaxDims; d++)
+        if (d < nd) a.gpay[g * kMaxDims + d] = pay.v[d];
+      uint32_t idx = atomicAdd(a.n_touched, 1u);
+      if (idx < a.touched_cap) a.touched[idx] = (uint32_t)g;
+    }
+    return;
+  }
   atomicAdd(&a.gsum[g], (unsigned long long)s);
   if (a.want_product) atomicAdd(&a.gprod[g], p);
   unsigned long long old = atomicAdd(&a.gcount[g], c);
@@ -717,16 +732,27 @@ __device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* pa
     if ((int64_t)i < a.out_cap) {
       GroupOut o;
       o.g = g;
-      o.sum = (long long)a.gsum[g];
-      o.count = a.gcount[g];
-      o.prod = a.gprod[g];
+      unsigned long long sum = a.gsum[g], cnt = a.gcount[g], prod = a.gprod[g];
+      for (int r = 1; r < a.g_reps; r++) {
+        const int64_t gi = (int64_t)g + r * a.g_stride;
+        sum += a.gsum[gi];
+        cnt += a.gcount[gi];
+        prod += a.gprod[gi];
+      }
+      o.sum = (long long)sum;
+      o.count = cnt;
+      o.prod = prod;
       for (int d = 0; d < kMaxDims; d++) o.pay[d] = a.gpay[(int64_t)g * kMaxDims + d];
       a.out[i] = o;
       if (a.host_hdr && (int64_t)i < a.host_groups) a.host_out[i] = o;
     }
-    a.gsum[g] = 0;
-    a.gcount[g] = 0;
-    a.gprod[g] = 0;
+    for (int r = 0; r < (a.g_reps > 1 ? a.g_reps : 1); r++) {
+      const int64_t gi = (int64_t)g + r * a.g_stride;
+      a.gsum[gi] = 0;
+      a.gcount[gi] = 0;
+      a.gprod[gi] = 0;
+    }
+    if (a.g_reps > 1) a.gseen[g] = 0;
   }
   for (int k = threadIdx.x; k < kStats; k += WG) {
     long long acc = 0;

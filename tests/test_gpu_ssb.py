@@ -4,6 +4,7 @@ oracle on the same seeded SF-1 tables.  Integer work: group keys, payloads,
 int64 sums, counts, per-stage line statistics and the reference's int32
 atomicAdd value are compared exactly."""
 import pytest
+import torch
 
 from lipjoin import ssb
 from lipjoin import workloads as W
@@ -13,6 +14,19 @@ from ssb_helpers import SF, brute_force, canon, oracle_query
 pytestmark = pytest.mark.gpu
 
 N_FACT = (1 << 20) + 77          # a ragged last tile
+
+
+@pytest.fixture(scope="module")
+def sctx():
+    """A context of its own: its group accumulators are sized by these queries'
+    group spaces (<= 65536 groups: eight copies, one per XCD), not by the
+    437,500-group Q3.3 space the shared context has seen."""
+    import lipjoin as L
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = L.Context(0)
+    yield c
+    c.close()
 
 
 @pytest.fixture(scope="module")
@@ -27,7 +41,8 @@ def _wrap32(x):
 
 
 @pytest.mark.parametrize("name", sorted(ssb.QUERIES))
-def test_ssb_query_all_plans(ctx, ssb_data, name):
+def test_ssb_query_all_plans(sctx, ssb_data, name):
+    ctx = sctx
     h, f, dev = ssb_data
     q = ssb.QUERIES[name]
     dims = ssb.dim_descs(q, dev, SF)
@@ -49,7 +64,8 @@ def test_ssb_query_all_plans(ctx, ssb_data, name):
 
 
 @pytest.mark.parametrize("name", ["q21", "q31", "q34", "q41"])
-def test_ssb_resident_dims(ctx, ssb_data, name):
+def test_ssb_resident_dims(sctx, ssb_data, name):
+    ctx = sctx
     """Tables built once (lj_dim_build) serve every plan of the query."""
     h, f, dev = ssb_data
     q = ssb.QUERIES[name]
@@ -63,3 +79,23 @@ def test_ssb_resident_dims(ctx, ssb_data, name):
     finally:
         for d in built:
             d.release()
+
+
+def test_group_copies_follow_the_group_space(ssb_data):
+    """A fresh context keeps eight accumulator copies for small group spaces and
+    one for a large one (re-sized when a larger space arrives); every result is
+    exact before and after the re-size."""
+    import lipjoin as L
+    h, f, dev = ssb_data
+    q = ssb.QUERIES["q31"]
+    want = canon(q, brute_force("q31", h, f))
+    with L.Context(0) as c:
+        for big in (False, True):
+            if big:
+                # a 437,500-group query (Q3.3's group space) over the same rows
+                q33 = ssb.QUERIES["q33"]
+                r = ssb.run(c, q33, dev, N_FACT, "lip", ssb.ORDERS[0], SF)
+                assert canon(q33, gpu_groups(r)) == canon(q33, brute_force("q33", h, f))
+            for method in ("naive", "lip-adaptive"):
+                res = ssb.run(c, q, dev, N_FACT, method, ssb.ORDERS[0], SF)
+                assert canon(q, gpu_groups(res)) == want, (big, method)
