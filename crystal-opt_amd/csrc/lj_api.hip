@@ -720,6 +720,9 @@ static lj_status validate_query(lj_ctx* c, const lj_star_desc* q, Plan& p) {
   if (q->n_groups < 1 || q->n_groups > (1ll << 26)) return fail(c, LJ_EINVAL, "n_groups %lld not in [1, 2^26]", (long long)q->n_groups);
   if (q->n_fact > 0) {
     if (!q->fact_measure || ((uintptr_t)q->fact_measure & 15)) return fail(c, LJ_EINVAL, "fact_measure must be a 16-byte aligned device pointer");
+    if ((uintptr_t)q->fact_measure_sub & 15) return fail(c, LJ_EINVAL, "fact_measure_sub must be 16-byte aligned");
+    if (q->fact_measure_sub && q->n_dims < 2)
+      return fail(c, LJ_EUNSUPPORTED, "fact_measure_sub needs two or more dimensions");
     for (int d = 0; d < nd; d++)
       if (!q->fact_fk[d] || ((uintptr_t)q->fact_fk[d] & 15))
         return fail(c, LJ_EINVAL, "fact_fk[%d] must be a 16-byte aligned device pointer", d);
@@ -817,6 +820,7 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   lj_status s = occupancy_of(c, cfg, p.lds_bytes, &p.per_cu);
   if (s != LJ_OK) return s;
   a.measure = q->fact_measure;
+  a.measure_sub = q->fact_measure_sub;
   a.n = q->n_fact;
   a.n_dims = nd;
   a.method = lip ? 1 : 0;

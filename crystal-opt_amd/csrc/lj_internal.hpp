@@ -78,6 +78,7 @@ struct ProbeDim {
 struct ProbeArgs {
   ProbeDim dim[kMaxDims];
   const int32_t* measure;
+  const int32_t* measure_sub;  // nullptr, or subtracted from measure per row (two or more dimensions)
   int64_t n;               // fact rows (< 2^31)
   int32_t n_dims;
   int32_t method;          // 0 NAIVE, 1 LIP

@@ -523,6 +523,7 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
         slot[t] = gl(reinterpret_cast<const uint64_t*>(D.ht))[alive && on && sl >= 0 ? sl : 0];
       }
       m = gl(a.measure)[alive ? (uint32_t)row : 0u];
+      if (a.measure_sub) m = (int32_t)((uint32_t)m - (uint32_t)gl(a.measure_sub)[alive ? (uint32_t)row : 0u]);
 #pragma unroll
       for (int t = 0; t < kMaxDims; t++) {
         if (t >= nd) continue;
@@ -534,7 +535,10 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
   }
   st.meas_rows += __popcll(__ballot(alive));
   st.meas_lines += batch_lines(alive, row, w.last_meas);
-  if (!LIP && alive) m = nd == 1 ? w.p_m : a.measure[(uint32_t)row];
+  if (!LIP && alive) {
+    m = nd == 1 ? w.p_m : a.measure[(uint32_t)row];
+    if (nd > 1 && a.measure_sub) m = (int32_t)((uint32_t)m - (uint32_t)a.measure_sub[(uint32_t)row]);
+  }
   st.v[ST_OOB] += group_add(a, gt, alive, pay, m);
   w.p_cnt = 0;
 }
@@ -651,7 +655,17 @@ __device__ __forceinline__ bool process_dense(const ProbeArgs& a, const uint32_t
   st.meas_rows += popc_ballot4(mask);
   v4i32 mv;
   if (nd == 1 && use_eager) mv = eager;
-  else mv = WavePredLoadVec4(gl(a.measure), row0, n, mask);
+  else {
+    mv = WavePredLoadVec4(gl(a.measure), row0, n, mask);
+    if (a.measure_sub) {
+      // Q4.x: revenue - supplycost, int32 wrap-around as the reference's int subtraction
+      const v4i32 sv = WavePredLoadVec4(gl(a.measure_sub), row0, n, mask);
+      mv.x = (int32_t)((uint32_t)mv.x - (uint32_t)sv.x);
+      mv.y = (int32_t)((uint32_t)mv.y - (uint32_t)sv.y);
+      mv.z = (int32_t)((uint32_t)mv.z - (uint32_t)sv.z);
+      mv.w = (int32_t)((uint32_t)mv.w - (uint32_t)sv.w);
+    }
+  }
 #pragma unroll
   for (int j = 0; j < 4; j++) st.v[ST_OOB] += group_add(a, gt, (mask >> j) & 1u, pay[j], vget(mv, j));
   return false;

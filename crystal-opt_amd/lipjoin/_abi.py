@@ -87,7 +87,7 @@ class StarDesc(C.Structure):
         ("bloom_order", C.c_int32 * MAX_DIMS), ("dims", DimDesc * MAX_DIMS),
         ("resident", C.c_void_p * MAX_DIMS), ("group_mult", C.c_int64 * MAX_DIMS),
         ("group_offset", C.c_int64), ("n_groups", C.c_int64), ("flags", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("reserved0", C.c_int32), ("fact_measure_sub", C.c_void_p),
     ]
 
 

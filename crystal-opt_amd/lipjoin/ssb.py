@@ -20,10 +20,8 @@ supplycost ~ U[1,10^5].
 Supplier's PHT_1 in Q2.x (q21/0-lip.cu:135, probed by BlockProbeAndPHT_1 at
 :77) is a PHT_2 here whose payload has group multiplier 0: with keys >= 1 both
 tables report a hit for exactly the inserted keys.  Q4.1 (q41/0-lip.cu) sums
-lo_revenue - lo_supplycost (:127-128): it runs as two probes of the same plan,
-one per measure column, and the per-group sums are subtracted.  Both probes
-select the same rows into the same groups, so the int64 difference equals the
-reference's int64 sum of the int32 differences exactly.
+(long long)(lo_revenue - lo_supplycost) (:127-128): the descriptor's
+fact_measure_sub column is gathered with the measure and subtracted per row.
 """
 import math
 from dataclasses import dataclass
@@ -223,23 +221,9 @@ def star(q, cols, n_fact, method, order, sf, bloom_order=None, dims=None, reside
 
 def run(ctx, q, cols, n_fact, method, order, sf, dims=None, resident=None, groups_cap=1 << 14):
     """lj_star_join of query q (Context.star_join's result dict).  A query with
-    a subtracted measure runs the plan twice, on lo_revenue and on the
-    subtracted column, and returns the per-group differences (times summed)."""
+    a subtracted measure (Q4.1) passes it as fact_measure_sub: the probe adds
+    (int32)(lo_revenue - lo_supplycost) per row, in one pass."""
     desc = star(q, cols, n_fact, method, order, sf, dims=dims, resident=resident)
-    res = ctx.star_join(desc, groups_cap=groups_cap)
-    if q.measure_sub is None:
-        return res
-    desc.fact_measure = _ptr(cols[q.measure_sub])
-    sub = ctx.star_join(desc, groups_cap=groups_cap)
-    if [(g[0], g[3]) for g in sub["groups"]] != [(g[0], g[3]) for g in res["groups"]]:
-        raise RuntimeError("the two measure passes selected different groups")
-    groups = []
-    for a, b in zip(res["groups"], sub["groups"]):
-        d = a[2] - b[2]
-        groups.append((a[0], a[1], d, a[3], (d + 2**31) % 2**32 - 2**31))
-    out = dict(res)
-    out["groups"] = groups
-    out["sum_total"] = res["sum_total"] - sub["sum_total"]
-    for k in ("time_query_ms", "time_build_ms", "time_probe_ms"):
-        out[k] = res[k] + sub[k]
-    return out
+    if q.measure_sub is not None:
+        desc.fact_measure_sub = _ptr(cols[q.measure_sub])
+    return ctx.star_join(desc, groups_cap=groups_cap)

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define LJ_ABI_VERSION 1
+#define LJ_ABI_VERSION 2
 #define LJ_MAX_DIMS 4
 #define LJ_MAX_PRED_STEPS 4
 
@@ -206,6 +206,10 @@ typedef struct {
   int64_t n_groups;                     /* dense group space, <= 2^26 */
   int32_t flags;                        /* 0 */
   int32_t reserved0;
+  const int32_t* fact_measure_sub;      /* device, or NULL.  Non-NULL: each row adds
+                                           (int32)(measure - measure_sub), as Q4.x's
+                                           revenue - supplycost (q41/0-lip.cu:127-128);
+                                           needs n_dims >= 2 and 16-byte alignment */
 } lj_star_desc;
 
 typedef struct {

@@ -99,3 +99,21 @@ def test_group_copies_follow_the_group_space(ssb_data):
             for method in ("naive", "lip-adaptive"):
                 res = ssb.run(c, q, dev, N_FACT, method, ssb.ORDERS[0], SF)
                 assert canon(q, gpu_groups(res)) == want, (big, method)
+
+
+def test_measure_sub_errors(sctx, ssb_data):
+    """fact_measure_sub needs two or more dimensions and 16-byte alignment."""
+    import lipjoin as L
+    h, f, dev = ssb_data
+    q = ssb.QUERIES["q41"]
+    desc = ssb.star(q, dev, N_FACT, "lip", (0, 1, 2, 3), SF)
+    desc.fact_measure_sub = dev["lo_supplycost"].data_ptr() + 4
+    with pytest.raises(L.LJError) as e:
+        sctx.star_join(desc)
+    assert e.value.status == 1
+    one = L.star_desc([dev["lo_suppkey"]], dev["lo_revenue"], N_FACT, ssb.dim_descs(q, dev, SF)[:1], (0,),
+                      method="lip", group_mult=(0,), n_groups=1)
+    one.fact_measure_sub = dev["lo_supplycost"].data_ptr()
+    with pytest.raises(L.LJError) as e:
+        sctx.star_join(one)
+    assert e.value.status == 5
