@@ -242,26 +242,25 @@ def bench_serve(ctx, n_req, fractions=(0.25, 0.5, 0.75, 0.9, 1.1, 1.5, 2.0)):
 
 
 def bench_ssb(ctx, n, reps=3, sf=20):
-    """SSB flights 2-4 (src/ssb-jo/q{21..34,41}) at n fact rows over SF-sf
+    """SSB flights 2-4 (src/ssb-jo/q{21..34,41..43}) at n fact rows over SF-sf
     tables: each query in its join orders x {naive, LIP}, and adaptive LIP.
-    Every plan's groups must agree (for Q4.1, whose group payloads outside the
-    group key are any row's, the keys, sums and counts); bytes_alg of a query is
-    the smallest over its LIP orders (128 B per touched line, the first column
-    in full; Q4.1's two measure passes are both counted by their timings, its
-    bytes_alg is one pass)."""
+    Every plan's group keys, sums and counts must agree (a payload outside the
+    group key is any kept row's value); bytes_alg of a query is the smallest
+    over its LIP orders (128 B per touched line, the first column in full; Q4.x
+    read the measure lines of both measure columns)."""
     import torch
     import lipjoin as L
     from lipjoin import ssb
     from lipjoin import workloads as W
-    # a context of its own: group accumulators sized for these group spaces
-    # (the Q3.3 space of the headline context is 437,500 groups)
-    ctx = L.Context(0)
+    # each query gets a context of its own, so its group accumulators are sized
+    # for its own group space (the Q3.3 space is 437,500 groups: one copy)
     log(f"ssb flights 2-4: generating {n} lineorder rows at SF {sf}")
     cols = ssb.device_columns(ctx, W.seed_of(7), n, sf)
     torch.cuda.synchronize()
     out = {}
     for name in sorted(ssb.QUERIES):
         q = ssb.QUERIES[name]
+        ctx = L.Context(0)
         dims = ssb.dim_descs(q, cols, sf)
         ref = None
         per = {}
@@ -282,6 +281,8 @@ def bench_ssb(ctx, n, reps=3, sf=20):
                     tp.append(r["time_probe_ms"])
                 st = res["stats"]
                 b = 128 * (sum(st["lines_in"][:len(q.dims) + 1]) + st["sample_lines"])
+                if q.measure_sub is not None:
+                    b += 128 * st["lines_in"][len(q.dims)]   # the second measure column, same lines
                 if method == "lip":
                     lip_bytes.append(b)
                 key = method if method == "lip-adaptive" else f"{oi}-{method}"
@@ -296,11 +297,11 @@ def bench_ssb(ctx, n, reps=3, sf=20):
                      "best_naive_query_ms": best_naive,
                      "worst_naive_query_ms": max(v["query_ms"] for k, v in per.items() if k.endswith("-naive")),
                      "best_lip_query_ms": min(v["query_ms"] for k, v in per.items() if k.endswith("-lip"))}
+        ctx.close()
         log(f"ssb {name}: adaptive {ad['query_ms']:.3f} ms (probe {ad['probe_ms']:.3f}, "
             f"frac {out[name]['adaptive_probe_hbm_frac']:.2f}), best naive {best_naive:.3f} ms, "
             f"{len(ref)} groups")
     del cols
-    ctx.close()
     torch.cuda.empty_cache()
     return out
 

@@ -165,6 +165,24 @@ QUERIES = {
                          DimQ("d", "lo_orderdate", None, "d_year", ())),
                  (0, 7, 0, 1), -1992, 7 * 25, 64, "SCPD",
                  "q41/0-lip.cu:120-128,151,157,178-186,209,217,237-243", measure_sub="lo_supplycost"),
+    # q42/0-naive.cu: S->C->P->D; g = ((year-1992)*25*25 + s_nation*25 + category) % (7*25*25)
+    "q42": Query("q42", (DimQ("s", "lo_suppkey", "s_region", "s_nation", ((COMB_SET, CMP_EQ, 1),)),
+                         DimQ("c", "lo_custkey", "c_region", "c_region", ((COMB_SET, CMP_EQ, 1),)),
+                         DimQ("p", "lo_partkey", "p_mfgr", "p_category",
+                              ((COMB_SET, CMP_EQ, 0), (COMB_OR, CMP_EQ, 1))),
+                         DimQ("d", "lo_orderdate", "d_year", "d_year",
+                              ((COMB_SET, CMP_EQ, 1997), (COMB_OR, CMP_EQ, 1998)))),
+                 (25, 0, 1, 625), -1992 * 625, 7 * 25 * 25, 64, "SCPD",
+                 "q42/0-naive.cu:94-101,123,150-152,181,210-213,254-268", measure_sub="lo_supplycost"),
+    # q43/q43.cu: S->C->P->D; g = ((year-1992)*250*1000 + s_city*1000 + brand) % (7*250*1000),
+    # int32 atomicAdd of revenue - supplycost
+    "q43": Query("q43", (DimQ("s", "lo_suppkey", "s_nation", "s_city", ((COMB_SET, CMP_EQ, 24),)),
+                         DimQ("c", "lo_custkey", "c_region", "c_region", ((COMB_SET, CMP_EQ, 1),)),
+                         DimQ("p", "lo_partkey", "p_category", "p_brand1", ((COMB_SET, CMP_EQ, 3),)),
+                         DimQ("d", "lo_orderdate", "d_year", "d_year",
+                              ((COMB_SET, CMP_EQ, 1997), (COMB_OR, CMP_EQ, 1998)))),
+                 (1000, 0, 1, 250 * 1000), -1992 * 250 * 1000, 7 * 250 * 1000, 32, "SCPD",
+                 "q43/q43.cu:105-111,133,160,189,218-221,262-276", measure_sub="lo_supplycost"),
     "q34": _flight3("q34", ("s_city", _CITIES), ("c_city", _CITIES),
                     ("d_yearmonthnum", ((COMB_SET, CMP_EQ, 199712),)), "city", 250, 32,
                     "q34/0-lip.cu:102-108,132-135,194-195"),
@@ -175,7 +193,8 @@ QUERIES = {
 # the descriptor dimensions (X, Y, D): 0 X->Y->D, 1 X->D->Y, 2 D->X->Y, 3 D->Y->X,
 # 4 Y->D->X, 5 Y->X->D, with X = supplier and Y = part (flight 2) or customer (flight 3).
 ORDERS = {0: (0, 1, 2), 1: (0, 2, 1), 2: (2, 0, 1), 3: (2, 1, 0), 4: (1, 2, 0), 5: (1, 0, 2)}
-# q41 ships one order, S->C->P->D (q41/0-{naive,lip}.cu line 1)
+# q41, q42 and q43 ship one order, S->C->P->D (q41/0-{naive,lip}.cu line 1,
+# q42/0-naive.cu line 1, the probe of q43/q43.cu:70-93)
 ORDERS_4 = {0: (0, 1, 2, 3)}
 
 

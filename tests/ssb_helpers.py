@@ -58,6 +58,10 @@ def brute_force(name, host_dims, host_fact):
     q34 c_city, s_city in {231,235}, d_yearmonthnum=199712
     q41 c_region=s_region=1, p_mfgr in {0,1}, SUM(revenue - supplycost)
                                             group (d_year, c_nation)
+    q42 c_region=s_region=1, p_mfgr in {0,1}, d_year in {1997,1998}
+                                            group (d_year, s_nation, p_category)
+    q43 c_region=1, s_nation=24, p_category=3, d_year in {1997,1998}
+                                            group (d_year, s_city, p_brand1)
     (d_year runs 1992..1998, so the reference's overwriting BlockPredLTE keeps
     1992..1997 as the SQL does)."""
     h, f = host_dims, host_fact
@@ -73,6 +77,23 @@ def brute_force(name, host_dims, host_fact):
         g = (nation * 7 + year - 1992) % 175
         pay = np.stack([region, nation, mfgr, year], axis=1)
         measure = f["lo_revenue"].astype(np.int64) - f["lo_supplycost"].astype(np.int64)
+        return _aggregate(g, pay, measure, keep)
+    if name in ("q42", "q43"):
+        c = _row_of("c", f["lo_custkey"], h)
+        p = _row_of("p", f["lo_partkey"], h)
+        c_region = h["c_region"][c].astype(np.int64)
+        years = (year == 1997) | (year == 1998)
+        measure = f["lo_revenue"].astype(np.int64) - f["lo_supplycost"].astype(np.int64)
+        if name == "q42":
+            nation, cat = h["s_nation"][s].astype(np.int64), h["p_category"][p].astype(np.int64)
+            keep = (h["s_region"][s] == 1) & (c_region == 1) & (h["p_mfgr"][p] <= 1) & years
+            g = ((year - 1992) * 625 + nation * 25 + cat) % 4375
+            pay = np.stack([nation, c_region, cat, year], axis=1)
+        else:
+            city, brand = h["s_city"][s].astype(np.int64), h["p_brand1"][p].astype(np.int64)
+            keep = (h["s_nation"][s] == 24) & (c_region == 1) & (h["p_category"][p] == 3) & years
+            g = ((year - 1992) * 250000 + city * 1000 + brand) % 1750000
+            pay = np.stack([city, c_region, brand, year], axis=1)
         return _aggregate(g, pay, measure, keep)
     if name.startswith("q2"):
         p = _row_of("p", f["lo_partkey"], h)
