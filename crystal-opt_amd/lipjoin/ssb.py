@@ -39,7 +39,10 @@ D_VAL_LEN = DATE_MAX - DATE_MIN + 1
 
 
 def table_sizes(sf):
-    """(customer, supplier, part, date) rows at scale factor sf."""
+    """(customer, supplier, part, date) rows at scale factor sf; a 4-tuple of
+    row counts (tables read from files) is returned as it is."""
+    if isinstance(sf, tuple):
+        return sf
     return 30000 * sf, 2000 * sf, 200000 * int(math.floor(1 + math.log2(sf))), D_LEN
 
 
@@ -221,7 +224,7 @@ def dim_descs(q, cols, sf, bloom=True):
     out = []
     for dq in q.dims:
         ht_len, kmin, bits = dim_params(dq, sf)
-        n_rows = D_LEN if dq.table == "d" else _ht_len(dq.table, sf)
+        n_rows = table_sizes(sf)[3] if dq.table == "d" else _ht_len(dq.table, sf)
         out.append(dim_desc(cols[f"{dq.table}_key"], n_rows, ht_len,
                             attr=cols[dq.attr] if dq.attr else None, payload=cols[dq.payload],
                             predicate=dq.pred, keys_min=kmin, bloom=("mod", bits) if bloom else None))
@@ -246,3 +249,103 @@ def run(ctx, q, cols, n_fact, method, order, sf, dims=None, resident=None, group
     if q.measure_sub is not None:
         desc.fact_measure_sub = _ptr(cols[q.measure_sub])
     return ctx.star_join(desc, groups_cap=groups_cap)
+
+
+# ------------------------------------------------------------------ the CLI --
+# Column files of the reference's loader (src/ssb/ssb_utils.h:106-153: table
+# name + column index, raw little-endian int32), by the names used here.
+COLUMN_FILES = {
+    "lo_custkey": "LINEORDER2", "lo_partkey": "LINEORDER3", "lo_suppkey": "LINEORDER4",
+    "lo_orderdate": "LINEORDER5", "lo_revenue": "LINEORDER12", "lo_supplycost": "LINEORDER13",
+    "p_key": "PART0", "p_mfgr": "PART2", "p_category": "PART3", "p_brand1": "PART4",
+    "s_key": "SUPPLIER0", "s_city": "SUPPLIER3", "s_nation": "SUPPLIER4", "s_region": "SUPPLIER5",
+    "c_key": "CUSTOMER0", "c_city": "CUSTOMER3", "c_nation": "CUSTOMER4", "c_region": "CUSTOMER5",
+    "d_key": "DDATE0", "d_year": "DDATE4", "d_yearmonthnum": "DDATE5",
+}
+
+# payload columns in the order the reference prints a result row (then the sum):
+# q21-q23 year brand (q21/0-lip.cu:281-283), q31-q34 year c_x s_x,
+# q41 year c_nation, q42 year s_nation category, q43 year s_city brand
+PRINT_ORDER = {"q21": (2, 1), "q22": (2, 1), "q23": (2, 1), "q31": (2, 1, 0), "q32": (2, 1, 0),
+               "q33": (2, 1, 0), "q34": (2, 1, 0), "q41": (3, 1), "q42": (3, 0, 2), "q43": (3, 0, 2)}
+
+
+def read_columns(data_dir, names):
+    """numpy int32 columns from the reference's column files; the row count of
+    each table is the file's size / 4."""
+    import os
+    out = {}
+    for k in names:
+        path = os.path.join(data_dir, COLUMN_FILES[k])
+        out[k] = np.fromfile(path, dtype="<i4")
+    return out
+
+
+def parse_plan(q, plan):
+    """(method, order) of a plan name: <i>-naive / <i>-lip (the reference's
+    binaries) or lip-adaptive."""
+    orders = orders_of(q)
+    if plan == "lip-adaptive":
+        return "lip-adaptive", orders[0]
+    i, _, m = plan.partition("-")
+    if not i.isdigit() or int(i) not in orders or m not in ("naive", "lip"):
+        raise ValueError(f"unknown plan {plan!r} for {q.name}")
+    return m, orders[int(i)]
+
+
+def main(argv=None):
+    import argparse
+    import time
+    ap = argparse.ArgumentParser(
+        prog="python -m lipjoin.ssb",
+        description="SSB join-order query over the C-ABI; a stand-in for the reference's "
+                    "bin/ssb-jo/<query>/<i>-<method> binaries (same --t, Result rows and "
+                    '{"query":N,"time_query":ms} lines).')
+    ap.add_argument("--query", required=True, choices=sorted(QUERIES))
+    ap.add_argument("--plan", default="0-lip", help="<i>-naive | <i>-lip | lip-adaptive")
+    ap.add_argument("--t", type=int, default=3, help="trials")
+    ap.add_argument("--data_dir", help="directory of the reference's column files")
+    ap.add_argument("--sf", type=int, default=20, help="scale factor of the synthetic tables")
+    ap.add_argument("--n", type=int, default=120_000_000, help="synthetic lineorder rows")
+    ap.add_argument("--i32", action="store_true", help="print the int32 (wrapping) sums")
+    ap.add_argument("--device", type=int, default=0)
+    args = ap.parse_args(argv)
+
+    import torch
+    from . import Context
+    from . import workloads as W
+    q = QUERIES[args.query]
+    method, order = parse_plan(q, args.plan)
+    ctx = Context(args.device)
+    if args.data_dir:
+        need = {dq.fk for dq in q.dims} | {f"{dq.table}_key" for dq in q.dims} | {"lo_revenue"}
+        need |= {dq.attr for dq in q.dims if dq.attr} | {dq.payload for dq in q.dims}
+        if q.measure_sub:
+            need.add(q.measure_sub)
+        host = read_columns(args.data_dir, sorted(need))
+        sizes = tuple(host[f"{t}_key"].size if f"{t}_key" in host else 0 for t in "cspd")
+        cols = {k: torch.from_numpy(v).to(f"cuda:{args.device}") for k, v in host.items()}
+        n, sf = host["lo_revenue"].size, sizes
+        print("** LOADED DATA **")
+    else:
+        cols = device_columns(ctx, W.seed_of(7), args.n, args.sf, f"cuda:{args.device}")
+        n, sf = args.n, args.sf
+        print("** GENERATED DATA **")
+    torch.cuda.synchronize()
+    print("** LOADED DATA TO GPU **")
+    for _ in range(args.t):
+        t0 = time.perf_counter()
+        r = run(ctx, q, cols, n, method, order, sf, groups_cap=1 << 20)
+        wall = (time.perf_counter() - t0) * 1e3
+        print("Result:")
+        for g, pay, s, _c, s32 in r["groups"]:
+            print(" ".join(str(pay[i]) for i in PRINT_ORDER[q.name]), s32 if args.i32 else s)
+        print(f"Res Count: {r['n_groups']}")
+        print(f"Time Taken Total: {wall:g}")
+        print(f'{{"query":{q.name[1:]},"time_query":{r["time_query_ms"]:g}}}')
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -117,3 +117,21 @@ def test_measure_sub_errors(sctx, ssb_data):
     with pytest.raises(L.LJError) as e:
         sctx.star_join(one)
     assert e.value.status == 5
+
+
+@pytest.mark.parametrize("name,plan", [("q31", "2-naive"), ("q22", "5-lip"), ("q43", "0-lip")])
+def test_cli_reads_reference_column_files(ssb_data, tmp_path, capsys, name, plan):
+    """python -m lipjoin.ssb over files in the reference's column format prints
+    the reference's result rows (payloads in its order, then the sum)."""
+    h, f, _ = ssb_data
+    q = ssb.QUERIES[name]
+    for k, fname in ssb.COLUMN_FILES.items():
+        src = h.get(k, f.get(k))
+        src.astype("<i4").tofile(tmp_path / fname)
+    assert ssb.main([f"--query={name}", f"--plan={plan}", "--t=1", f"--data_dir={tmp_path}"]) == 0
+    out = capsys.readouterr().out.splitlines()
+    rows = out[out.index("Result:") + 1: next(i for i, l in enumerate(out) if l.startswith("Res Count"))]
+    want = [" ".join(str(g[1][i]) for i in ssb.PRINT_ORDER[name]) + f" {g[2]}"
+            for g in canon(q, brute_force(name, h, f))]
+    assert rows == want
+    assert out[-1].startswith('{"query":%s,"time_query":' % name[1:])

@@ -50,3 +50,20 @@ def test_q41_other_join_order(data):
     for method in ("naive", "lip"):
         got = oracle_query(q, h, f, (3, 2, 0, 1), method)[0]
         assert canon(q, got) == canon(q, brute_force("q41", h, f))
+
+
+def test_column_files_and_plan_names(tmp_path):
+    """The CLI reads the reference's column files (ssb_utils.h naming) and its plan names."""
+    h = ssb.host_dims(W.seed_of(6), SF)
+    h["s_key"].astype("<i4").tofile(tmp_path / "SUPPLIER0")
+    h["s_region"].astype("<i4").tofile(tmp_path / "SUPPLIER5")
+    got = ssb.read_columns(str(tmp_path), ["s_key", "s_region"])
+    assert (got["s_key"] == h["s_key"]).all() and (got["s_region"] == h["s_region"]).all()
+    assert ssb.parse_plan(ssb.QUERIES["q33"], "3-lip") == ("lip", ssb.ORDERS[3])
+    assert ssb.parse_plan(ssb.QUERIES["q42"], "0-naive") == ("naive", (0, 1, 2, 3))
+    assert ssb.parse_plan(ssb.QUERIES["q21"], "lip-adaptive")[0] == "lip-adaptive"
+    for bad in ("6-lip", "1-lip", "x", "0-bloom"):
+        q = ssb.QUERIES["q41" if bad == "1-lip" else "q21"]
+        with pytest.raises(ValueError):
+            ssb.parse_plan(q, bad)
+    assert set(ssb.PRINT_ORDER) == set(ssb.QUERIES)
