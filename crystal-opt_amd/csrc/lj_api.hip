@@ -810,6 +810,23 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
     cfg.k0 = kind_of(0);
     for (int d = 1; d < nd; d++)
       if (kind_of(d) != cfg.k0) cfg.k0 = S0_GENERIC;
+    if (cfg.k0 == S0_GENERIC) {
+      // filters of mixed kinds or placements (some do not fit in LDS): stage 0
+      // is kept to the staged filters when they share one kind, so the kernel
+      // stays specialised; the sample orders those first, the others after
+      int k = -1;
+      bool one = true;
+      for (int d = 0; d < nd; d++)
+        if (a.dim[d].bloom_lds >= 0) {
+          if (k < 0) k = kind_of(d);
+          else if (kind_of(d) != k) one = false;
+        }
+      if (k >= 0 && one) {
+        cfg.k0 = k;
+        for (int d = 0; d < nd; d++)
+          if (kind_of(d) == k) a.s0_mask |= 1u << d;
+      }
+    }
   }
   // every filter of the kind (and placement) of stage 0: the kernel probes
   // later stages with that kind's short test
@@ -829,6 +846,18 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   for (int i = 0; i < LJ_MAX_DIMS; i++) {
     a.order[i] = i < nd ? q->order[i] : i;
     a.load_order[i] = i < nd ? (lip ? p.bloom_order[i] : q->order[i]) : i;
+  }
+  if (a.s0_mask) {
+    // the order a wave starts from (and keeps if it samples no row): the
+    // dimensions allowed at stage 0 first, each group in its given order
+    int32_t lo[LJ_MAX_DIMS];
+    int k = 0;
+    for (int pass = 0; pass < 2; pass++)
+      for (int i = 0; i < nd; i++) {
+        const bool ok = (a.s0_mask >> a.load_order[i]) & 1u;
+        if (ok == (pass == 0)) lo[k++] = a.load_order[i];
+      }
+    for (int i = 0; i < nd; i++) a.load_order[i] = lo[i];
   }
   a.lds_filter_words = lds_words;
   a.goffset = q->group_offset;

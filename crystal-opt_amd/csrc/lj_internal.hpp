@@ -88,6 +88,8 @@ struct ProbeArgs {
   int32_t load_order[kMaxDims];   // column load order (NAIVE: order, LIP: Bloom order)
   int32_t lds_filter_words;
   int32_t all_k0;          // LIP: every dimension's filter has stage 0's kind and placement
+  uint32_t s0_mask;        // adaptive: dimensions the sample may put first (their filters have
+                           // stage 0's kind); 0 = any
   int64_t goffset;
   int64_t n_groups;
   unsigned long long* gsum;

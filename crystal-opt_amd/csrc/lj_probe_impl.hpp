@@ -940,10 +940,14 @@ __device__ __forceinline__ void sample_tile(const ProbeArgs& a, const uint32_t* 
     tot[kMaxDims] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(rows_old + n_valid));
   }
   if (tot[kMaxDims] == 0) return;
-  // rank of each dimension by pass count (ties: lower index first)
+  // rank of each dimension by pass count (ties: lower index first); with a
+  // stage-0 mask, the dimensions outside it rank after every one inside it
   uint32_t cntd[kMaxDims];
 #pragma unroll
-  for (int d = 0; d < kMaxDims; d++) cntd[d] = d < w.nd ? tot[d] : 0xFFFFFFFFu;
+  for (int d = 0; d < kMaxDims; d++) {
+    const uint32_t late = (a.s0_mask != 0u && !((a.s0_mask >> d) & 1u)) ? 0x80000000u : 0u;
+    cntd[d] = d < w.nd ? (tot[d] | late) : 0xFFFFFFFFu;
+  }
   int rank[kMaxDims];
 #pragma unroll
   for (int d = 0; d < kMaxDims; d++) {
@@ -962,7 +966,7 @@ __device__ __forceinline__ void sample_tile(const ProbeArgs& a, const uint32_t* 
   w.ld1 = uniform(o[1]);
   w.ld2 = uniform(o[2]);
   w.ld3 = uniform(o[3]);
-  const uint32_t c0 = o[0] == 0 ? cntd[0] : o[0] == 1 ? cntd[1] : o[0] == 2 ? cntd[2] : cntd[3];
+  const uint32_t c0 = (o[0] == 0 ? cntd[0] : o[0] == 1 ? cntd[1] : o[0] == 2 ? cntd[2] : cntd[3]) & 0x7FFFFFFFu;
   eager = (uint64_t)c0 * 4 > tot[kMaxDims];
 }
 
