@@ -15,9 +15,10 @@ constexpr int kGroupSlots = 32;      // LDS-privatised groups per workgroup
 // Group spaces up to kRepGroups keep kGroupReps copies of the global group
 // accumulators (copy = workgroup index mod 8, i.e. one per XCD under the
 // round-robin dispatch), so that groups overflowing the LDS table are not all
-// added into the same few cache lines by every CU.
+// added into the same few cache lines by every CU.  The bound keeps the
+// finalize (one workgroup reads every copy of every touched group) short.
 constexpr int kGroupReps = 8;
-constexpr int64_t kRepGroups = 1 << 16;
+constexpr int64_t kRepGroups = 1 << 13;
 constexpr int kQueue = 64;           // survivor rows per batch (one per lane)
 constexpr int kQueueCap = kQueue + 8; // survivor queue entries per wave
 constexpr int kStats = 16;           // per-workgroup statistics words

@@ -747,7 +747,10 @@ __device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* pa
       GroupOut o;
       o.g = g;
       unsigned long long sum = a.gsum[g], cnt = a.gcount[g], prod = a.gprod[g];
-      for (int r = 1; r < a.g_reps; r++) {
+      // every copy's loads go out together (one round trip)
+#pragma unroll
+      for (int r = 1; r < kGroupReps; r++) {
+        if (r >= a.g_reps) continue;
         const int64_t gi = (int64_t)g + r * a.g_stride;
         sum += a.gsum[gi];
         cnt += a.gcount[gi];

@@ -19,7 +19,7 @@ N_FACT = (1 << 20) + 77          # a ragged last tile
 @pytest.fixture(scope="module")
 def sctx():
     """A context of its own: its group accumulators are sized by these queries'
-    group spaces (<= 65536 groups: eight copies, one per XCD), not by the
+    group spaces (<= 8192 groups: eight copies, one per XCD), not by the
     437,500-group Q3.3 space the shared context has seen."""
     import lipjoin as L
     if not torch.cuda.is_available():
