@@ -31,6 +31,8 @@ def sctx():
 
 @pytest.fixture(scope="module")
 def ssb_data():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
     seed = W.seed_of(6)
     h, f = ssb.host_dims(seed, SF), ssb.host_fact(seed, N_FACT, SF)
     return h, f, to_device({**h, **f})
@@ -135,3 +137,28 @@ def test_cli_reads_reference_column_files(ssb_data, tmp_path, capsys, name, plan
             for g in canon(q, brute_force(name, h, f))]
     assert rows == want
     assert out[-1].startswith('{"query":%s,"time_query":' % name[1:])
+
+
+def test_ssb_batch_matches_brute_force(sctx, ssb_data):
+    """One lj_batch over SSB queries of three and four dimensions (Q4.x with the
+    second measure column, adaptive plans with the stage-0 mask): every
+    query's groups equal its brute force."""
+    h, f, dev = ssb_data
+    plans = [("q21", "lip", 0), ("q31", "lip-adaptive", 0), ("q41", "lip-adaptive", 0), ("q42", "naive", 0),
+             ("q43", "lip", 0), ("q43", "lip-adaptive", 0), ("q33", "lip", 3)]
+    descs = []
+    for name, method, oi in plans:
+        q = ssb.QUERIES[name]
+        d = ssb.star(q, dev, N_FACT, method, ssb.orders_of(q)[oi], SF)
+        if q.measure_sub:
+            d.fact_measure_sub = dev[q.measure_sub].data_ptr()
+        descs.append(d)
+    b = sctx.batch(descs, groups_cap=1 << 12)
+    try:
+        for rep in range(2):
+            res = b.run()
+            for (name, method, oi), r in zip(plans, res):
+                q = ssb.QUERIES[name]
+                assert canon(q, gpu_groups(r)) == canon(q, brute_force(name, h, f)), (rep, name, method)
+    finally:
+        b.close()
