@@ -350,8 +350,9 @@ def main():
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the star join shards by fact "
-              "rows with replicated dimensions, see DESIGN.md.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}). The star join would shard "
+              "by fact rows, with the dimension tables and filters replicated on every GPU and one "
+              "int64 all-reduce of the group sums; that path is not built.", file=sys.stderr)
         sys.exit(2)
 
     import torch
