@@ -186,7 +186,27 @@ def bench_serve(ctx, n_req, fractions=(0.25, 0.5, 0.75, 0.9, 1.1, 1.5, 2.0)):
     reqs = W.serve_requests(seed, n_req, len(tmpls), n_pool)
     descs = [W.serve_desc(pool, tmpls[t], o, n) for (t, o, n) in reqs]
     rows = [n for (_, _, n) in reqs]
-    ref = [ctx.star_join(d) for d in descs[:64]]      # warm-up; reference results
+    # every request's result alone (also the warm-up): (status, rows, sum)
+    ref = []
+    for d in descs:
+        st = ctx.star_join_raw(d)
+        ref.append((st, ctx._raw_res.n_rows, ctx._raw_res.sum_total))
+    # a sample of them against the oracle's single join on the same slice
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    keys, attr = cols["keys"].cpu().numpy(), cols["attr"].cpu().numpy()
+    fk_h, meas_h = pool["fk"][:n_pool].cpu().numpy(), pool["meas"][:n_pool].cpu().numpy()
+    oracle_ok, oracle_n = True, 0
+    for i in range(0, n_req, max(1, n_req // 16)):
+        t_i, o, nr = reqs[i]
+        b = tmpls[t_i]["bloom"]
+        bs = (O.bloom_spec(O.BLOOM_MOD, b[1], 1) if b[0] == "mod" else
+              O.bloom_spec(O.BLOOM_MULT, b[1], b[2], layout=b[3]))
+        spec = O.DimSpec(keys, attr, keys.size, pred=list(W.single_join_pred(tmpls[t_i]["threshold"])), bloom=bs)
+        r_o = O.single_join(spec, fk_h[o:o + nr], meas_h[o:o + nr], method=O.LIP, threads=os.cpu_count() or 1)
+        oracle_ok = oracle_ok and ref[i] == (0, r_o["n_match"], r_o["sum_measure"])
+        oracle_n += 1
+    del fk_h, meas_h
     t = []
     for d in descs[:min(n_req, 500)]:
         t0 = time.perf_counter()
@@ -195,13 +215,14 @@ def bench_serve(ctx, n_req, fractions=(0.25, 0.5, 0.75, 0.9, 1.1, 1.5, 2.0)):
     service = statistics.fmean(t)
     capacity = 1.0 / service
     out = {"requests_per_rate": n_req, "serial_service_ms": service * 1e3, "serial_capacity_req_s": capacity,
+           "hbm_budget_bytes": 200 << 30, "oracle_sample": oracle_n, "oracle_sample_match": oracle_ok,
            "rates": []}
     rng = np.random.default_rng(seed + 7)
     for f in fractions:
         lam = f * capacity
         arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
         # serving loop
-        srv = ctx.serve(max_batch=64, max_inflight=2)
+        srv = ctx.serve(hbm_budget=200 << 30, max_batch=64, max_inflight=2)
         lag = []
         t0 = time.perf_counter()
         ids = []
@@ -215,7 +236,8 @@ def bench_serve(ctx, n_req, fractions=(0.25, 0.5, 0.75, 0.9, 1.1, 1.5, 2.0)):
         st = srv.stats()
         srv.stop()
         lat = [r["time_query_ms"] + 1e3 * lg for r, lg in zip(res, lag)]
-        exact = all((r["sum_total"], r["n_rows"]) == (a["sum_total"], a["n_rows"]) for r, a in zip(res, ref))
+        # every request: served result == the same request alone
+        exact = all(r is not None and (r["status"], r["n_rows"], r["sum_total"]) == a for r, a in zip(res, ref))
         # one query at a time, same arrivals
         lat_s = []
         t0 = time.perf_counter()
@@ -230,7 +252,7 @@ def bench_serve(ctx, n_req, fractions=(0.25, 0.5, 0.75, 0.9, 1.1, 1.5, 2.0)):
              "served_rows_per_s": sum(rows) / wall, "launches": st["launches"],
              "mean_requests_per_launch": st["completed"] / max(st["launches"], 1),
              "serial_p50_ms": _pct(lat_s, 50), "serial_p99_ms": _pct(lat_s, 99),
-             "serial_rows_per_s": sum(rows) / wall_s, "bit_exact": exact,
+             "serial_rows_per_s": sum(rows) / wall_s, "bit_exact": exact, "checked": len(res),
              "submit_lag_p99_ms": _pct(lag, 99) * 1e3}
         out["rates"].append(e)
         log(f"serve x{f}: served p50 {e['served_p50_ms']:.3f} p99 {e['served_p99_ms']:.3f} ms "
@@ -304,6 +326,24 @@ def bench_ssb(ctx, n, reps=3, sf=20):
     del cols
     torch.cuda.empty_cache()
     return out
+
+
+def host_cpu_share():
+    """(threads to use, CPUs the OS shows): every CPU this process may run on
+    (its affinity set), capped by the cgroup's CPU quota when one is set."""
+    visible = os.cpu_count() or 1
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = visible
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return n, visible
 
 
 def cpu_baseline(n_fact, seed, threads):
@@ -483,11 +523,12 @@ def main():
     if not args.no_single:
         out["single_join"] = bench_single(ctx)
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads, visible = host_cpu_share()
         log(f"cpu baseline on {threads} threads")
         cb, cpu_groups = cpu_baseline(n, seed, threads)
         gpu_groups = [(g, tuple(p), s, c) for (g, p, s, c, _) in r["groups"]]
         cb["matches_gpu"] = cpu_groups == gpu_groups
+        cb["host_cpus_visible"] = visible
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -48,6 +48,10 @@ struct lj_dim {
   int64_t n_selected = -1;
   int64_t n_oob = -1;
   float time_build_ms = 0.f;
+  // tables taken from the zero pool (cleared behind their last use) go back
+  // there; tables cleared inside the query's own window (lj_hash_join, the
+  // reference's timed memsets) go back to the plain free list
+  bool zero_pool = true;
 };
 
 constexpr int kEvents = 4;
@@ -463,9 +467,15 @@ static void dims_free(lj_ctx* c, lj_dim* const* dims, int n) {
   for (int i = 0; i < n; i++) {
     lj_dim* D = dims[i];
     if (!D) continue;
-    dev_release_zero(c, D->ht, f);
-    dev_release_zero(c, D->bloom, f);
-    dev_release_zero(c, D->counters, f);
+    if (D->zero_pool) {
+      dev_release_zero(c, D->ht, f);
+      dev_release_zero(c, D->bloom, f);
+      dev_release_zero(c, D->counters, f);
+    } else {
+      dev_release(c, D->ht);
+      dev_release(c, D->bloom);
+      dev_release(c, D->counters);
+    }
     delete D;
   }
   if (f.n) (void)launch_fill(f, c->stream);
@@ -482,23 +492,35 @@ static void sum_counters(const unsigned long long* h, unsigned long long out[2])
   }
 }
 
-// Allocate the tables of a dimension (no launches).
-static lj_status dim_alloc(lj_ctx* c, const lj_dim_desc* desc, lj_dim** out) {
+// Allocate the tables of a dimension (no launches): all-zero tables from the
+// zero pool, or (zero_pool false) uncleared ones that the caller clears.
+static lj_status dim_alloc(lj_ctx* c, const lj_dim_desc* desc, lj_dim** out, bool zero_pool = true) {
   lj_status s = check_dim_desc(c, desc);
   if (s != LJ_OK) return s;
   lj_dim* D = new lj_dim();
   D->desc = *desc;
+  D->zero_pool = zero_pool;
   if (!D->desc.payload) D->desc.payload = D->desc.attr;
   dim_hashes(D);
-  if ((s = dev_alloc_zero_t(c, D->ht_words, &D->ht)) != LJ_OK ||
-      (D->bloom_words && (s = dev_alloc_zero_t(c, D->bloom_words, &D->bloom)) != LJ_OK) ||
-      (s = dev_alloc_zero_t(c, kCounterWords, &D->counters)) != LJ_OK) {
+  if (zero_pool) {
+    if ((s = dev_alloc_zero_t(c, D->ht_words, &D->ht)) != LJ_OK ||
+        (D->bloom_words && (s = dev_alloc_zero_t(c, D->bloom_words, &D->bloom)) != LJ_OK) ||
+        (s = dev_alloc_zero_t(c, kCounterWords, &D->counters)) != LJ_OK) {
+      dim_free(c, D);
+      return s;
+    }
+  } else if ((s = dev_alloc_t(c, D->ht_words, &D->ht)) != LJ_OK ||
+             (D->bloom_words && (s = dev_alloc_t(c, D->bloom_words, &D->bloom)) != LJ_OK) ||
+             (s = dev_alloc_t(c, kCounterWords, &D->counters)) != LJ_OK) {
     dim_free(c, D);
     return s;
   }
   *out = D;
   return LJ_OK;
 }
+
+// 32-bit words of a dimension's counter block (kCounterWords 64-bit counters).
+constexpr int64_t kCounterFillWords = (int64_t)kCounterWords * 2;
 
 static void add_fill(FillArgs& f, void* p, int64_t words) {
   if (!p || words <= 0) return;
@@ -731,14 +753,15 @@ static lj_status validate_query(lj_ctx* c, const lj_star_desc* q, Plan& p) {
   return LJ_OK;
 }
 
-// Resident dimensions, or tables allocated (all zero) for a build inside the call.
-static lj_status acquire_dims(lj_ctx* c, const lj_star_desc* q, Plan& p) {
+// Resident dimensions, or tables allocated for a build inside the call (all
+// zero, or with zero_pool false uncleared: the caller queues their clear).
+static lj_status acquire_dims(lj_ctx* c, const lj_star_desc* q, Plan& p, bool zero_pool = true) {
   lj_status s = LJ_OK;
   for (int d = 0; d < p.nd && s == LJ_OK; d++) {
     if (q->resident[d]) {
       p.dims[d] = const_cast<lj_dim*>(q->resident[d]);
     } else {
-      s = dim_alloc(c, &q->dims[d], &p.dims[d]);
+      s = dim_alloc(c, &q->dims[d], &p.dims[d], zero_pool);
       if (s == LJ_OK) p.owned[p.n_owned++] = p.dims[d];
     }
   }
@@ -891,13 +914,18 @@ static lj_status parse_result(lj_ctx* c, const lj_star_desc* q, const Plan& p, c
   return LJ_OK;
 }
 
-// The common path of lj_star_join and lj_hash_join.
-static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, QueryOut& qo) {
+// The common path of lj_star_join and lj_hash_join.  With fill_in_window
+// (lj_hash_join) the query's tables are cleared inside its own timed window,
+// as the reference's hashJoin times its memsets (bloom_join.cu:135-137), and
+// that clear is t_fill; otherwise they come from the zero pool, cleared by a
+// fill queued behind the query that last used them (t_fill = 0).
+static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, QueryOut& qo,
+                           bool fill_in_window = false) {
   Plan p;
   lj_status s = validate_query(c, q, p);
   if (s != LJ_OK) return s;
   HIP_TRY(c, hipSetDevice(c->device));
-  if ((s = acquire_dims(c, q, p)) != LJ_OK) return s;
+  if ((s = acquire_dims(c, q, p, !fill_in_window)) != LJ_OK) return s;
   auto cleanup = [&]() {
     dims_free(c, p.owned, p.n_owned);
     p.n_owned = 0;
@@ -911,6 +939,17 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   // goes out first; the probe is planned while it runs.
   const bool built_here = p.n_owned > 0;
   HIP_TRY(c, hipEventRecord(c->ev[0], c->stream));
+  const bool timed_fill = fill_in_window && built_here;
+  if (timed_fill) {
+    FillArgs f{};
+    for (int i = 0; i < p.n_owned; i++) {
+      add_fill(f, p.owned[i]->ht, p.owned[i]->ht_words);
+      add_fill(f, p.owned[i]->bloom, p.owned[i]->bloom_words);
+      add_fill(f, p.owned[i]->counters, kCounterFillWords);
+    }
+    HIP_TRY(c, launch_fill(f, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev[2], c->stream));
+  }
   if (built_here) {
     if ((s = dims_launch(c, p.owned, p.n_owned, c->out_hdr + kClockSlot)) != LJ_OK) {
       cleanup();
@@ -969,8 +1008,17 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
   cleanup();
   HIP_TRY(c, spin_wait(c->ev[kEvDone]));
   const int64_t n_touched = hdr[0];
-  if (n_touched > (int64_t)c->touched_cap)
+  if (n_touched > (int64_t)c->touched_cap) {
+    // the finalize zeroed only the listed groups: clear the rest so that the
+    // next query on this context starts from all-zero accumulators
+    FillArgs f{};
+    add_fill(f, c->gsum, c->g_cap * c->g_reps * 2);
+    add_fill(f, c->gcount, c->g_cap * c->g_reps * 2);
+    add_fill(f, c->gprod, c->g_cap * c->g_reps * 2);
+    if (c->g_reps > 1) add_fill(f, c->gseen, c->g_cap);
+    HIP_TRY(c, launch_fill(f, c->stream));
     return fail(c, LJ_EUNSUPPORTED, "%lld groups were hit; at most %u can be listed", (long long)n_touched, c->touched_cap);
+  }
   std::vector<GroupOut> big;
   const GroupOut* groups = hg;
   if (n_touched > first) {
@@ -984,9 +1032,14 @@ static lj_status run_query(lj_ctx* c, const lj_star_desc* q, int want_product, Q
       groups = big.data();
     }
   }
-  // phases from HIP events on the query's stream: [build launch] [probe launch]
+  // phases from HIP events on the query's stream: [fill] [build launch] [probe launch]
   qo.t_fill = 0.f;
-  HIP_TRY(c, hipEventElapsedTime(&qo.t_build, c->ev[0], c->ev[3]));
+  if (timed_fill) {
+    HIP_TRY(c, hipEventElapsedTime(&qo.t_fill, c->ev[0], c->ev[2]));
+    HIP_TRY(c, hipEventElapsedTime(&qo.t_build, c->ev[2], c->ev[3]));
+  } else {
+    HIP_TRY(c, hipEventElapsedTime(&qo.t_build, c->ev[0], c->ev[3]));
+  }
   HIP_TRY(c, hipEventElapsedTime(&qo.t_probe, c->ev[3], c->ev[1]));
   HIP_TRY(c, hipEventElapsedTime(&qo.t_total, c->ev[0], c->ev[1]));
   if (!built_here) qo.t_build = 0.f;
@@ -1027,7 +1080,7 @@ static void to_star_result(const lj_star_desc* q, const QueryOut& qo, lj_star_re
   out->time_query_ms = qo.t_total;
   out->time_build_ms = qo.t_fill + qo.t_build;
   out->time_probe_ms = qo.t_probe;
-  out->reserved0 = 0;
+  out->status = LJ_OK;
   fill_stats(qo, q->n_dims, &out->stats);
 }
 
@@ -1035,8 +1088,12 @@ extern "C" lj_status lj_star_join(lj_ctx* c, const lj_star_desc* q, lj_star_resu
   if (!c || !q || !out) return fail(c, LJ_EINVAL, "NULL argument");
   QueryOut qo;
   lj_status s = run_query(c, q, 0, qo);
-  if (s != LJ_OK && s != LJ_EDOMAIN) return s;
+  if (s != LJ_OK && s != LJ_EDOMAIN) {
+    out->status = s;
+    return s;
+  }
   to_star_result(q, qo, out);
+  out->status = s;
   return s;
 }
 
@@ -1056,7 +1113,7 @@ extern "C" lj_status lj_hash_join(lj_ctx* c, const lj_join_desc* j, lj_join_resu
   q.dims[0] = j->dim;
   q.n_groups = 1;
   QueryOut qo;
-  lj_status s = run_query(c, &q, 1, qo);
+  lj_status s = run_query(c, &q, 1, qo, true);
   if (s != LJ_OK && s != LJ_EDOMAIN) return s;
   memset(out, 0, sizeof *out);
   out->n_selected = qo.dim_selected[0];
@@ -1146,6 +1203,37 @@ struct BatchKind {
   uint32_t* d_wgb = nullptr;
 };
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// A query's buffers in a batch or a serving launch, in three regions:
+//   zero  : gsum | gcount | gprod [ng each] | n_touched, done (kept zero
+//           between runs: the finalize clears what the query touched)
+//   plain : gpay [ng * 4] | touched [tcap] | statistics rows [rows * kStats]
+//   out   : result header | groups [tcap]
+struct QBufLayout {
+  size_t zero = 0, plain = 0, out = 0;
+};
+QBufLayout qbuf_layout(int64_t ng, int64_t tcap, int64_t stats_rows) {
+  QBufLayout l;
+  l.zero = align256((size_t)ng * 8) * 3 + 256;
+  l.plain = align256((size_t)ng * kMaxDims * 4) + align256((size_t)tcap * 4) + align256((size_t)stats_rows * kStats * 8);
+  l.out = align256(kHdrBytes + sizeof(GroupOut) * (size_t)tcap);
+  return l;
+}
+void qbuf_bind(ProbeArgs& a, int64_t ng, int64_t tcap, char* z, char* pl, char* od) {
+  const size_t g8 = align256((size_t)ng * 8);
+  a.gsum = reinterpret_cast<unsigned long long*>(z);
+  a.gcount = reinterpret_cast<unsigned long long*>(z + g8);
+  a.gprod = reinterpret_cast<unsigned long long*>(z + 2 * g8);
+  a.n_touched = reinterpret_cast<uint32_t*>(z + 3 * g8);
+  a.done = a.n_touched + 32;
+  a.gpay = reinterpret_cast<int32_t*>(pl);
+  a.touched = reinterpret_cast<uint32_t*>(pl + align256((size_t)ng * kMaxDims * 4));
+  a.stats = reinterpret_cast<long long*>(pl + align256((size_t)ng * kMaxDims * 4) + align256((size_t)tcap * 4));
+  a.touched_cap = (uint32_t)tcap;
+  a.out_hdr = reinterpret_cast<long long*>(od);
+  a.out = reinterpret_cast<GroupOut*>(od + kHdrBytes);
+  a.out_cap = tcap;
+}
 }  // namespace
 
 struct lj_batch {
@@ -1271,13 +1359,13 @@ static lj_status batch_build(lj_ctx* c, const lj_star_desc* descs, int32_t n, co
   for (BatchQuery& bq : b->q) {
     const int64_t ng = bq.desc.n_groups;
     bq.tcap = std::min<int64_t>(ng, b->opts.groups_cap);
+    const QBufLayout l = qbuf_layout(ng, bq.tcap, bq.grid);
     bq.zero_off = zero_bytes;
-    zero_bytes += align256((size_t)ng * 8) * 3 + 256;
+    zero_bytes += l.zero;
     bq.plain_off = plain_bytes;
-    plain_bytes += align256((size_t)ng * kMaxDims * 4) + align256((size_t)bq.tcap * 4) +
-                   align256((size_t)bq.grid * kStats * 8);
+    plain_bytes += l.plain;
     bq.out_off = out_bytes;
-    out_bytes += align256(kHdrBytes + sizeof(GroupOut) * (size_t)bq.tcap);
+    out_bytes += l.out;
   }
   lj_status s;
   void* zp = nullptr;
@@ -1330,21 +1418,8 @@ static lj_status batch_build(lj_ctx* c, const lj_star_desc* descs, int32_t n, co
     for (size_t j = 0; j < K.qs.size(); j++) {
       BatchQuery& bq = b->q[(size_t)K.qs[j]];
       ProbeArgs a = bq.p.a;
-      const int64_t ng = bq.desc.n_groups;
-      char* z = b->zero_buf + bq.zero_off;
-      a.gsum = reinterpret_cast<unsigned long long*>(z);
-      a.gcount = reinterpret_cast<unsigned long long*>(z + align256((size_t)ng * 8));
-      a.gprod = reinterpret_cast<unsigned long long*>(z + 2 * align256((size_t)ng * 8));
-      a.n_touched = reinterpret_cast<uint32_t*>(z + 3 * align256((size_t)ng * 8));
-      a.done = a.n_touched + 32;
-      char* pl = b->plain_buf + bq.plain_off;
-      a.gpay = reinterpret_cast<int32_t*>(pl);
-      a.touched = reinterpret_cast<uint32_t*>(pl + align256((size_t)ng * kMaxDims * 4));
-      a.stats = reinterpret_cast<long long*>(pl + align256((size_t)ng * kMaxDims * 4) + align256((size_t)bq.tcap * 4));
-      a.touched_cap = (uint32_t)bq.tcap;
-      a.out_hdr = reinterpret_cast<long long*>(b->out_dev + bq.out_off);
-      a.out = reinterpret_cast<GroupOut*>(b->out_dev + bq.out_off + kHdrBytes);
-      a.out_cap = bq.tcap;
+      qbuf_bind(a, bq.desc.n_groups, bq.tcap, b->zero_buf + bq.zero_off, b->plain_buf + bq.plain_off,
+                b->out_dev + bq.out_off);
       hq[j] = a;
       hw[j] = begin;
       begin += (uint32_t)bq.grid;
@@ -1366,7 +1441,7 @@ static lj_status batch_build(lj_ctx* c, const lj_star_desc* descs, int32_t n, co
     for (int i = 0; i < bq.p.n_owned; i++) {
       lj_dim* D = bq.p.owned[i];
       void* ps[3] = {D->ht, D->bloom, D->counters};
-      int64_t ws[3] = {D->ht_words, D->bloom_words, kCounterWords};
+      int64_t ws[3] = {D->ht_words, D->bloom_words, kCounterFillWords};
       for (int t = 0; t < 3; t++) {
         if (!ps[t] || ws[t] <= 0) continue;
         if (f.n == kMaxFill) {
@@ -1434,26 +1509,40 @@ static lj_status batch_launch(lj_batch* b) {
   return LJ_OK;
 }
 
-// After e1 has completed: the results of the last run.
-static lj_status batch_collect(lj_batch* b, lj_star_result* results) {
+// After e1 has completed: the results of the last run.  status (optional)
+// receives each query's own status; the return value is the first failure.
+static lj_status batch_collect(lj_batch* b, lj_star_result* results, lj_status* status = nullptr) {
   lj_ctx* c = b->c;
   lj_status s;
   float ms = 0.f;
   HIP_TRY(c, hipEventElapsedTime(&ms, b->e0, b->e1));
   lj_status first_err = LJ_OK;
+  FillArgs f{};
   for (size_t i = 0; i < b->q.size(); i++) {
     BatchQuery& bq = b->q[i];
     const long long* hdr = reinterpret_cast<const long long*>(b->out_host + bq.out_off);
     const GroupOut* groups = reinterpret_cast<const GroupOut*>(b->out_host + bq.out_off + kHdrBytes);
     QueryOut qo;
     s = parse_result(c, &bq.desc, bq.p, hdr, groups, bq.tcap, qo);
-    if (s == LJ_OK && hdr[0] > bq.tcap)
-      s = fail(c, LJ_EUNSUPPORTED, "query %zu hit %lld groups; groups_cap is %lld", i, hdr[0], (long long)bq.tcap);
+    if (hdr[0] > bq.tcap) {
+      // the finalize zeroed only the listed groups: clear the query's whole
+      // accumulator region before the next run
+      if (f.n == kMaxFill) {
+        HIP_TRY(c, launch_fill(f, b->stream));
+        f = FillArgs{};
+      }
+      add_fill(f, b->zero_buf + bq.zero_off, (int64_t)(align256((size_t)bq.desc.n_groups * 8) * 3 / 4));
+      if (s == LJ_OK)
+        s = fail(c, LJ_EUNSUPPORTED, "query %zu hit %lld groups; groups_cap is %lld", i, hdr[0], (long long)bq.tcap);
+    }
     qo.t_total = qo.t_probe = ms;
     qo.t_build = qo.t_fill = 0.f;
     to_star_result(&bq.desc, qo, &results[i]);
+    results[i].status = s;
+    if (status) status[i] = s;
     if (s != LJ_OK && first_err == LJ_OK) first_err = s;
   }
+  if (f.n) HIP_TRY(c, launch_fill(f, b->stream));
   return first_err;
 }
 
@@ -1645,14 +1734,43 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
 }
 
 // ------------------------------------------------------------------- serving --
+// The loop keeps everything a launch needs: per launch slot a stream, two
+// events and the argument buffers (pinned staging + device copy); per request
+// a buffer set from a pool (group accumulators kept zero by the finalize,
+// statistics rows, the device result region and a pinned host result region
+// that the finalize writes directly).  A launch of one request is the
+// single-query probe (arguments in the kernarg segment, no upload); a launch
+// of several uploads their arguments once and runs the batched probe.
 namespace {
 using Clock = std::chrono::steady_clock;
+
+// Statistics rows per request buffer: the most workgroups one query is given
+// (n_cu x the most workgroups of a configuration per CU).
+constexpr int kServeWgPerCu = 8;
+// The loop spins this long for new work before sleeping on the condition.
+constexpr auto kServeIdleSpin = std::chrono::milliseconds(2);
+
+struct ReqBuf {
+  int64_t ng = 0;       // group-space class (a power of two >= n_groups)
+  int64_t tcap = 0;     // groups listed
+  QBufLayout l;
+  char* dev = nullptr;  // zero | plain | out
+  char* host = nullptr; // pinned: header | groups [tcap]
+  char* host_dev = nullptr;  // the same memory, as the device addresses it
+  size_t host_bytes = 0;
+  uint64_t bytes = 0;   // device bytes (the budget's unit)
+  bool fresh = true;    // zero region not cleared yet
+};
 
 struct ServeReq {
   uint64_t id = 0;
   lj_star_desc desc{};
   Clock::time_point t_submit;
   uint64_t bytes = 0;
+  // admission
+  Plan p;
+  ReqBuf* buf = nullptr;
+  int64_t grid = 0;
 };
 
 struct ServeDone {
@@ -1660,15 +1778,25 @@ struct ServeDone {
   std::string err;
   lj_star_result r{};
   std::vector<lj_group> groups;
-  int32_t n_dims = 0;
 };
 
-struct InFlight {
-  lj_batch* b = nullptr;
+struct ServeSlot {
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  char* args_dev = nullptr;   // ProbeArgs [max_batch] | wg_begin [max_batch + S0_NKINDS]
+  char* args_host = nullptr;  // pinned staging of args_dev
+  size_t args_bytes = 0;
+  size_t wgb_off = 0;
+  bool busy = false;
   std::vector<ServeReq> reqs;
   uint64_t bytes = 0;
-  int slot = 0;
 };
+
+int64_t pow2_at_least(int64_t x) {
+  int64_t v = 1;
+  while (v < x) v <<= 1;
+  return v;
+}
 }  // namespace
 
 struct lj_server {
@@ -1684,18 +1812,76 @@ struct lj_server {
   uint64_t next_id = 1;
   bool stopping = false;
   lj_serve_stats stats{};
-  std::vector<hipStream_t> streams;
-  std::vector<bool> slot_busy;
+  std::vector<ServeSlot> slots;
+  std::multimap<int64_t, ReqBuf*> free_bufs;  // idle request buffers by group class
+  std::vector<ReqBuf*> all_bufs;
+  int64_t stats_rows = 0;
 };
 
-// Device memory a request holds while in flight (the batch's per-query
-// buffers): group accumulators and payloads, result region, statistics rows.
-static uint64_t serve_bytes(const lj_ctx* c, const lj_star_desc* q, int64_t groups_cap) {
-  const uint64_t ng = (uint64_t)q->n_groups;
-  const uint64_t tcap = std::min<uint64_t>(ng, (uint64_t)groups_cap);
-  const uint64_t slots = (uint64_t)c->n_cu * 2;
-  return align256(ng * 8) * 3 + 256 + align256(ng * kMaxDims * 4) + align256(tcap * 4) +
-         align256(slots * kStats * 8) + align256(kHdrBytes + sizeof(GroupOut) * tcap) + sizeof(ProbeArgs);
+// Device bytes of a request buffer for n_groups (the serving budget's unit).
+static uint64_t serve_bytes(const lj_server* S, int64_t n_groups) {
+  const int64_t ng = pow2_at_least(n_groups);
+  const int64_t tcap = std::min<int64_t>(ng, S->cfg.groups_cap);
+  const QBufLayout l = qbuf_layout(ng, tcap, S->stats_rows);
+  return l.zero + l.plain + l.out;
+}
+
+static void reqbuf_free(lj_ctx* c, ReqBuf* b) {
+  dev_release(c, b->dev);
+  if (b->host) (void)hipHostFree(b->host);
+  delete b;
+}
+
+// An idle buffer of n_groups' class, or a new one (its zero region is cleared
+// on the stream of its first launch).
+static lj_status reqbuf_get(lj_server* S, int64_t n_groups, ReqBuf** out) {
+  lj_ctx* c = S->c;
+  const int64_t ng = pow2_at_least(n_groups);
+  auto it = S->free_bufs.find(ng);
+  if (it != S->free_bufs.end()) {
+    *out = it->second;
+    S->free_bufs.erase(it);
+    return LJ_OK;
+  }
+  ReqBuf* b = new ReqBuf();
+  b->ng = ng;
+  b->tcap = std::min<int64_t>(ng, S->cfg.groups_cap);
+  b->l = qbuf_layout(ng, b->tcap, S->stats_rows);
+  b->bytes = b->l.zero + b->l.plain + b->l.out;
+  void* d = nullptr;
+  lj_status s = dev_alloc(c, b->bytes, &d);
+  if (s != LJ_OK) {
+    delete b;
+    return s;
+  }
+  b->dev = static_cast<char*>(d);
+  b->host_bytes = kHdrBytes + sizeof(GroupOut) * (size_t)b->tcap;
+  void* h = nullptr;
+  void* hd = nullptr;
+  if (hipHostMalloc(&h, b->host_bytes, hipHostMallocDefault) != hipSuccess ||
+      hipHostGetDevicePointer(&hd, h, 0) != hipSuccess) {
+    if (h) (void)hipHostFree(h);
+    b->host = nullptr;
+    reqbuf_free(c, b);
+    return fail(c, LJ_EHIP, "pinned result buffer allocation failed");
+  }
+  b->host = static_cast<char*>(h);
+  b->host_dev = static_cast<char*>(hd);
+  S->all_bufs.push_back(b);
+  *out = b;
+  return LJ_OK;
+}
+
+// The request's arguments: its plan with its buffers; the header and groups
+// also go to the pinned host region.
+static ProbeArgs serve_args(const ServeReq& rq) {
+  ProbeArgs a = rq.p.a;
+  ReqBuf* b = rq.buf;
+  qbuf_bind(a, b->ng, b->tcap, b->dev, b->dev + b->l.zero, b->dev + b->l.zero + b->l.plain);
+  a.host_hdr = reinterpret_cast<long long*>(b->host_dev);
+  a.host_out = reinterpret_cast<GroupOut*>(b->host_dev + kHdrBytes);
+  a.host_groups = b->tcap;
+  return a;
 }
 
 static void serve_finish(lj_server* S, const ServeReq& rq, lj_status st, const char* err, const lj_star_result* r,
@@ -1703,72 +1889,194 @@ static void serve_finish(lj_server* S, const ServeReq& rq, lj_status st, const c
   ServeDone d;
   d.st = st;
   if (err) d.err = err;
-  d.n_dims = rq.desc.n_dims;
   if (r) {
     d.r = *r;
     const int64_t n = std::min<int64_t>(r->n_groups, r->groups_cap);
     d.groups.assign(groups, groups + std::max<int64_t>(n, 0));
   }
   d.r.groups = nullptr;
+  d.r.status = st;
   d.r.time_query_ms = (float)std::chrono::duration<double, std::milli>(t_done - rq.t_submit).count();
   S->done[rq.id] = std::move(d);
+}
+
+// Plan the admitted requests and queue their launch on the slot's stream.
+// Requests that fail to plan finish at once with their own status (out).
+static lj_status serve_launch(lj_server* S, ServeSlot& sl, std::vector<std::pair<ServeReq, lj_status>>& failed) {
+  lj_ctx* c = S->c;
+  std::vector<ServeReq> ok;
+  for (ServeReq& rq : sl.reqs) {
+    lj_status s = validate_query(c, &rq.desc, rq.p);
+    if (s == LJ_OK) s = acquire_dims(c, &rq.desc, rq.p);
+    if (s == LJ_OK) s = plan_probe(c, &rq.desc, 0, 0, rq.p);
+    if (s == LJ_OK) s = reqbuf_get(S, rq.desc.n_groups, &rq.buf);
+    if (s != LJ_OK) {
+      failed.emplace_back(std::move(rq), s);
+      continue;
+    }
+    ok.push_back(std::move(rq));
+  }
+  sl.reqs = std::move(ok);
+  if (sl.reqs.empty()) return LJ_OK;
+  // clear the zero regions of new buffers first (stream order)
+  FillArgs f{};
+  for (ServeReq& rq : sl.reqs)
+    if (rq.buf->fresh) {
+      if (f.n == kMaxFill) {
+        HIP_TRY(c, launch_fill(f, sl.stream));
+        f = FillArgs{};
+      }
+      add_fill(f, rq.buf->dev, (int64_t)(rq.buf->l.zero / 4));
+      rq.buf->fresh = false;
+    }
+  if (f.n) HIP_TRY(c, launch_fill(f, sl.stream));
+  HIP_TRY(c, hipEventRecord(sl.e0, sl.stream));
+  if (sl.reqs.size() == 1) {
+    ServeReq& rq = sl.reqs[0];
+    const ProbeConfig& cfg = rq.p.cfg;
+    const int64_t tile = (int64_t)cfg.vec * kSegRows * (cfg.wg / 64);
+    const int64_t need = std::max<int64_t>(1, (rq.desc.n_fact + tile - 1) / tile);
+    rq.grid = std::min<int64_t>({(int64_t)c->n_cu * rq.p.per_cu, need, S->stats_rows});
+    const ProbeArgs a = serve_args(rq);
+    HIP_TRY(c, launch_probe(cfg, a, (int)rq.grid, rq.p.lds_bytes, sl.stream));
+  } else {
+    // group by kernel configuration; one upload, one launch per configuration
+    std::vector<std::vector<size_t>> kinds;
+    std::vector<ProbeConfig> kcfg;
+    std::vector<size_t> klds;
+    for (size_t i = 0; i < sl.reqs.size(); i++) {
+      const ProbeConfig& cf = sl.reqs[i].p.cfg;
+      size_t k = 0;
+      while (k < kcfg.size() && !(kcfg[k].wg == cf.wg && kcfg[k].vec == cf.vec && kcfg[k].k0 == cf.k0)) k++;
+      if (k == kcfg.size()) {
+        kcfg.push_back(cf);
+        kinds.emplace_back();
+        klds.push_back(0);
+      }
+      kinds[k].push_back(i);
+      klds[k] = std::max(klds[k], sl.reqs[i].p.lds_bytes);
+    }
+    ProbeArgs* hq = reinterpret_cast<ProbeArgs*>(sl.args_host);
+    uint32_t* hw = reinterpret_cast<uint32_t*>(sl.args_host + sl.wgb_off);
+    std::vector<size_t> q0(kinds.size()), w0(kinds.size());
+    std::vector<int64_t> kgrid(kinds.size(), 0);
+    size_t qi = 0, wi = 0;
+    for (size_t k = 0; k < kinds.size(); k++) {
+      int per_cu = 0;
+      lj_status s = occupancy_of(c, kcfg[k], klds[k], &per_cu);
+      if (s != LJ_OK) return s;
+      const int64_t tile = (int64_t)kcfg[k].vec * kSegRows * (kcfg[k].wg / 64);
+      q0[k] = qi;
+      w0[k] = wi;
+      uint32_t begin = 0;
+      for (size_t i : kinds[k]) {
+        ServeReq& rq = sl.reqs[i];
+        const int64_t need = std::max<int64_t>(1, (rq.desc.n_fact + tile - 1) / tile);
+        rq.grid = std::min<int64_t>({(int64_t)c->n_cu * per_cu, need, S->stats_rows});
+        hq[qi++] = serve_args(rq);
+        hw[wi++] = begin;
+        begin += (uint32_t)rq.grid;
+      }
+      hw[wi++] = begin;
+      kgrid[k] = begin;
+    }
+    HIP_TRY(c, hipMemcpyAsync(sl.args_dev, sl.args_host, sl.args_bytes, hipMemcpyHostToDevice, sl.stream));
+    const ProbeArgs* dq = reinterpret_cast<const ProbeArgs*>(sl.args_dev);
+    const uint32_t* dw = reinterpret_cast<const uint32_t*>(sl.args_dev + sl.wgb_off);
+    for (size_t k = 0; k < kinds.size(); k++)
+      HIP_TRY(c, launch_probe_multi(kcfg[k], dq + q0[k], dw + w0[k], (int)kinds[k].size(), (int)kgrid[k], klds[k],
+                                    sl.stream));
+  }
+  HIP_TRY(c, hipEventRecord(sl.e1, sl.stream));
+  return LJ_OK;
+}
+
+// A completed launch: every request's result from its pinned host region.
+static void serve_complete(lj_server* S, ServeSlot& sl) {
+  lj_ctx* c = S->c;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, sl.e0, sl.e1);
+  const Clock::time_point t_done = Clock::now();
+  bool dirty = false;
+  std::vector<ServeDone> outs(sl.reqs.size());
+  for (size_t j = 0; j < sl.reqs.size(); j++) {
+    ServeReq& rq = sl.reqs[j];
+    ReqBuf* b = rq.buf;
+    const long long* hdr = reinterpret_cast<const long long*>(b->host);
+    const GroupOut* groups = reinterpret_cast<const GroupOut*>(b->host + kHdrBytes);
+    QueryOut qo;
+    lj_status s = parse_result(c, &rq.desc, rq.p, hdr, groups, b->tcap, qo);
+    if (hdr[0] > b->tcap) {
+      // the finalize zeroed only the listed groups: clear the rest before reuse
+      FillArgs f{};
+      add_fill(f, b->dev, (int64_t)(b->l.zero / 4));
+      (void)launch_fill(f, sl.stream);
+      dirty = true;
+      if (s == LJ_OK)
+        s = fail(c, LJ_EUNSUPPORTED, "request hit %lld groups; groups_cap is %lld", hdr[0], (long long)b->tcap);
+    }
+    qo.t_total = qo.t_probe = ms;
+    qo.t_build = qo.t_fill = 0.f;
+    ServeDone& d = outs[j];
+    d.groups.resize((size_t)b->tcap);
+    d.r.groups = d.groups.data();
+    d.r.groups_cap = b->tcap;
+    to_star_result(&rq.desc, qo, &d.r);
+    d.r.status = s;
+    d.st = s;
+    if (s != LJ_OK) d.err = c->err;
+  }
+  if (dirty) (void)hipStreamSynchronize(sl.stream);  // the clears, before the buffers are reused
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (size_t j = 0; j < sl.reqs.size(); j++) {
+    ServeReq& rq = sl.reqs[j];
+    serve_finish(S, rq, outs[j].st, outs[j].st == LJ_OK ? nullptr : outs[j].err.c_str(), &outs[j].r,
+                 outs[j].groups.data(), t_done);
+    S->free_bufs.emplace(rq.buf->ng, rq.buf);
+    rq.buf = nullptr;
+  }
+  S->stats.completed += sl.reqs.size();
+  S->stats.bytes_in_flight -= sl.bytes;
+  sl.reqs.clear();
+  sl.bytes = 0;
+  sl.busy = false;
 }
 
 static void serve_loop(lj_server* S) {
   lj_ctx* c = S->c;
   (void)hipSetDevice(c->device);
-  std::vector<InFlight> inflight;
-  lj_batch_opts bopts{};
-  bopts.groups_cap = S->cfg.groups_cap;
   for (;;) {
     // 1. completions (the loop alone drives the context: no lock for HIP work)
-    for (size_t i = 0; i < inflight.size();) {
-      InFlight& f = inflight[i];
-      if (hipEventQuery(f.b->e1) != hipSuccess) {
-        i++;
+    bool any_busy = false, completed = false;
+    for (ServeSlot& sl : S->slots) {
+      if (!sl.busy) continue;
+      if (hipEventQuery(sl.e1) != hipSuccess) {
+        any_busy = true;
         continue;
       }
-      const size_t n = f.reqs.size();
-      std::vector<lj_star_result> res(n);
-      std::vector<std::vector<lj_group>> gs(n, std::vector<lj_group>((size_t)S->cfg.groups_cap));
-      for (size_t j = 0; j < n; j++) {
-        res[j].groups = gs[j].data();
-        res[j].groups_cap = S->cfg.groups_cap;
-      }
-      const lj_status st = batch_collect(f.b, res.data());
-      const std::string err = st == LJ_OK ? std::string() : c->err;
-      batch_free(f.b);
-      const Clock::time_point t_done = Clock::now();
-      {
-        std::lock_guard<std::mutex> lk(S->mu);
-        for (size_t j = 0; j < n; j++)
-          serve_finish(S, f.reqs[j], st, st == LJ_OK ? nullptr : err.c_str(), &res[j], gs[j].data(), t_done);
-        S->stats.completed += n;
-        S->stats.bytes_in_flight -= f.bytes;
-        S->slot_busy[(size_t)f.slot] = false;
-      }
-      S->cv_done.notify_all();
-      inflight.erase(inflight.begin() + (long)i);
+      serve_complete(S, sl);
+      completed = true;
     }
-    // 2. admission: every waiting request the budget admits, into one launch per free slot
-    std::vector<ServeReq> take;
-    int slot = -1;
-    uint64_t take_bytes = 0;
-    if (inflight.empty()) {
-      // idle: spin briefly for the next request before sleeping on the condition
-      const Clock::time_point until = Clock::now() + std::chrono::microseconds(200);
+    if (completed) S->cv_done.notify_all();
+    // 2. admission: every waiting request the budget admits, into one launch on a free slot
+    ServeSlot* slot = nullptr;
+    for (ServeSlot& sl : S->slots)
+      if (!sl.busy) slot = &sl;
+    if (!any_busy && S->n_pending.load(std::memory_order_acquire) == 0) {
+      // idle: spin for the next request before sleeping on the condition
+      const Clock::time_point until = Clock::now() + kServeIdleSpin;
       while (S->n_pending.load(std::memory_order_acquire) == 0 && Clock::now() < until) std::this_thread::yield();
     }
+    std::vector<ServeReq> take;
+    uint64_t take_bytes = 0;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      if (inflight.empty() && S->pending.empty()) {
+      if (!any_busy && S->pending.empty()) {
         if (S->stopping) break;
         S->cv_work.wait(lk, [&] { return S->stopping || !S->pending.empty(); });
         if (S->pending.empty()) break;
       }
-      for (size_t k = 0; k < S->slot_busy.size(); k++)
-        if (!S->slot_busy[k]) slot = (int)k;
-      if (slot >= 0) {
+      if (slot) {
         const uint64_t budget = S->cfg.hbm_budget;
         while (!S->pending.empty() && (int)take.size() < S->cfg.max_batch) {
           ServeReq& rq = S->pending.front();
@@ -1777,18 +2085,16 @@ static void serve_loop(lj_server* S) {
                          nullptr, Clock::now());
             S->stats.rejected++;
             S->pending.pop_front();
-            S->n_pending.store(S->pending.size(), std::memory_order_release);
             S->cv_done.notify_all();
             continue;
           }
           if (budget && S->stats.bytes_in_flight + take_bytes + rq.bytes > budget) break;
           take_bytes += rq.bytes;
-          take.push_back(rq);
+          take.push_back(std::move(rq));
           S->pending.pop_front();
         }
         S->n_pending.store(S->pending.size(), std::memory_order_release);
         if (!take.empty()) {
-          S->slot_busy[(size_t)slot] = true;
           S->stats.bytes_in_flight += take_bytes;
           S->stats.peak_bytes_in_flight = std::max(S->stats.peak_bytes_in_flight, S->stats.bytes_in_flight);
           S->stats.launches++;
@@ -1797,33 +2103,47 @@ static void serve_loop(lj_server* S) {
       }
     }
     if (!take.empty()) {
-      std::vector<lj_star_desc> descs;
-      descs.reserve(take.size());
-      for (const ServeReq& rq : take) descs.push_back(rq.desc);
-      lj_batch* b = nullptr;
-      lj_status st = batch_build(c, descs.data(), (int32_t)descs.size(), &bopts, S->streams[(size_t)slot], &b);
-      if (st == LJ_OK) st = batch_launch(b);
+      slot->reqs = std::move(take);
+      slot->bytes = take_bytes;
+      slot->busy = true;
+      std::vector<std::pair<ServeReq, lj_status>> failed;
+      lj_status st = serve_launch(S, *slot, failed);
       if (st != LJ_OK) {
-        const std::string err = c->err;
-        if (b) batch_free(b);
-        std::lock_guard<std::mutex> lk(S->mu);
-        for (const ServeReq& rq : take) serve_finish(S, rq, st, err.c_str(), nullptr, nullptr, Clock::now());
-        S->stats.rejected += take.size();
-        S->stats.bytes_in_flight -= take_bytes;
-        S->slot_busy[(size_t)slot] = false;
-        S->cv_done.notify_all();
-      } else {
-        InFlight f;
-        f.b = b;
-        f.reqs = std::move(take);
-        f.bytes = take_bytes;
-        f.slot = slot;
-        inflight.push_back(std::move(f));
+        // the launch itself failed: every request of it fails with that status
+        for (ServeReq& rq : slot->reqs) failed.emplace_back(std::move(rq), st);
+        slot->reqs.clear();
       }
+      if (!failed.empty()) {
+        const std::string err = c->err;
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (auto& fr : failed) {
+          if (fr.first.buf) S->free_bufs.emplace(fr.first.buf->ng, fr.first.buf);
+          serve_finish(S, fr.first, fr.second, err.c_str(), nullptr, nullptr, Clock::now());
+          S->stats.rejected++;
+          S->stats.bytes_in_flight -= fr.first.bytes;
+          slot->bytes -= fr.first.bytes;
+        }
+        S->cv_done.notify_all();
+      }
+      if (slot->reqs.empty()) slot->busy = false;
       continue;
     }
-    if (!inflight.empty()) std::this_thread::yield();  // poll the launches in flight
+    if (any_busy) std::this_thread::yield();  // poll the launches in flight
   }
+}
+
+static void server_free(lj_server* S) {
+  lj_ctx* c = S->c;
+  for (ServeSlot& sl : S->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.e0) (void)hipEventDestroy(sl.e0);
+    if (sl.e1) (void)hipEventDestroy(sl.e1);
+    dev_release(c, sl.args_dev);
+    if (sl.args_host) (void)hipHostFree(sl.args_host);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  for (ReqBuf* b : S->all_bufs) reqbuf_free(c, b);
+  delete S;
 }
 
 extern "C" lj_status lj_serve_start(lj_ctx* c, const lj_serve_cfg* cfg, lj_server** out) {
@@ -1840,16 +2160,26 @@ extern "C" lj_status lj_serve_start(lj_ctx* c, const lj_serve_cfg* cfg, lj_serve
     delete S;
     return fail(c, LJ_EINVAL, "max_batch <= 4096 and max_inflight <= 8");
   }
-  for (int i = 0; i < S->cfg.max_inflight; i++) {
-    hipStream_t st = nullptr;
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-      for (hipStream_t s2 : S->streams) (void)hipStreamDestroy(s2);
-      delete S;
-      return fail(c, LJ_EHIP, "hipStreamCreate failed");
+  S->stats_rows = (int64_t)c->n_cu * kServeWgPerCu;
+  S->slots.resize((size_t)S->cfg.max_inflight);
+  const size_t nb = (size_t)S->cfg.max_batch;
+  for (ServeSlot& sl : S->slots) {
+    sl.wgb_off = align256(sizeof(ProbeArgs) * nb);
+    sl.args_bytes = sl.wgb_off + align256(sizeof(uint32_t) * (nb + S0_NKINDS));
+    void* d = nullptr;
+    lj_status s = LJ_OK;
+    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.e0, hipEventDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.e1, hipEventDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&sl.args_host), sl.args_bytes, hipHostMallocDefault) != hipSuccess)
+      s = fail(c, LJ_EHIP, "serving slot setup failed");
+    if (s == LJ_OK) s = dev_alloc(c, sl.args_bytes, &d);
+    sl.args_dev = static_cast<char*>(d);
+    if (s != LJ_OK) {
+      server_free(S);
+      return s;
     }
-    S->streams.push_back(st);
   }
-  S->slot_busy.assign((size_t)S->cfg.max_inflight, false);
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   S->th = std::thread(serve_loop, S);
   *out = S;
@@ -1857,20 +2187,24 @@ extern "C" lj_status lj_serve_start(lj_ctx* c, const lj_serve_cfg* cfg, lj_serve
 }
 
 extern "C" lj_status lj_serve_submit(lj_server* S, const lj_star_desc* q, uint64_t* id) {
-  if (!S || !q || !id) return LJ_EINVAL;
-  for (int d = 0; d < q->n_dims && d < LJ_MAX_DIMS; d++)
-    if (!q->resident[d]) return LJ_EINVAL;  // serving runs over resident dimensions only
-  if (q->n_dims < 1 || q->n_dims > LJ_MAX_DIMS) return LJ_EINVAL;
+  if (!S || !q || !id) return fail(nullptr, LJ_EINVAL, "NULL argument");
+  // checked here, on the caller's thread (its message: lj_last_error(NULL) on
+  // this thread), so that one bad request never shares a launch
+  Plan p;
+  lj_status st = validate_query(nullptr, q, p);
+  if (st != LJ_OK) return st;
+  for (int d = 0; d < q->n_dims; d++)
+    if (!q->resident[d]) return fail(nullptr, LJ_EINVAL, "serving needs resident dimensions (dimension %d is not)", d);
   ServeReq rq;
   rq.desc = *q;
   rq.t_submit = Clock::now();
-  rq.bytes = serve_bytes(S->c, q, S->cfg.groups_cap);
+  rq.bytes = serve_bytes(S, q->n_groups);
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    if (S->stopping) return LJ_ESTATE;
+    if (S->stopping) return fail(nullptr, LJ_ESTATE, "the server is stopping");
     rq.id = S->next_id++;
     *id = rq.id;
-    S->pending.push_back(rq);
+    S->pending.push_back(std::move(rq));
     S->n_pending.store(S->pending.size(), std::memory_order_release);
     S->stats.submitted++;
   }
@@ -1915,7 +2249,6 @@ extern "C" lj_status lj_serve_stop(lj_server* S) {
   }
   S->cv_work.notify_all();
   if (S->th.joinable()) S->th.join();
-  for (hipStream_t st : S->streams) (void)hipStreamDestroy(st);
-  delete S;
+  server_free(S);
   return LJ_OK;
 }

@@ -260,7 +260,7 @@ def _star_result_dict(r, arr, n_dims, cap):
     return {
         "groups": groups, "n_groups": r.n_groups, "n_rows": r.n_rows, "sum_total": r.sum_total,
         "time_query_ms": r.time_query_ms, "time_build_ms": r.time_build_ms,
-        "time_probe_ms": r.time_probe_ms, "stats": stats_dict(r.stats),
+        "time_probe_ms": r.time_probe_ms, "stats": stats_dict(r.stats), "status": r.status,
     }
 
 
@@ -287,9 +287,13 @@ class Batch:
             self._res[i].groups = C.cast(self._arrs[i], C.POINTER(_abi.Group))
             self._res[i].groups_cap = self.cap
 
-    def run(self, allow_domain=False):
+    def run(self, allow_domain=False, allow_errors=False):
+        """Results of every query (each dict carries its own "status").  A
+        failed query raises unless allow_errors (allow_domain: EDOMAIN only)."""
         st = lib().lj_batch_run(self.h, self._res)
-        self.ctx._check(st, allow=(_abi.EDOMAIN,) if allow_domain else ())
+        allow = (_abi.EDOMAIN,) if allow_domain else ()
+        if not allow_errors or st in (_abi.EINVAL, _abi.EHIP):
+            self.ctx._check(st, allow=allow)
         return [_star_result_dict(self._res[i], self._arrs[i], self._n_dims[i], self.cap) for i in range(self.n)]
 
     def close(self):
@@ -324,7 +328,7 @@ class Server:
         i = C.c_uint64()
         st = lib().lj_serve_submit(self.h, C.byref(desc), C.byref(i))
         if st != _abi.OK:
-            raise LJError(st, "lj_serve_submit")
+            raise LJError(st, "lj_serve_submit: " + lib().lj_last_error(None).decode())
         self._n_dims[i.value] = desc.n_dims
         return i.value
 

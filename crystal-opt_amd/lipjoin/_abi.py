@@ -102,7 +102,7 @@ class StarResult(C.Structure):
     _fields_ = [
         ("groups", C.POINTER(Group)), ("groups_cap", C.c_int64), ("n_groups", C.c_int64),
         ("n_rows", C.c_int64), ("sum_total", C.c_int64), ("time_query_ms", C.c_float),
-        ("time_build_ms", C.c_float), ("time_probe_ms", C.c_float), ("reserved0", C.c_float),
+        ("time_build_ms", C.c_float), ("time_probe_ms", C.c_float), ("status", C.c_int32),
         ("stats", ProbeStats),
     ]
 

@@ -180,8 +180,9 @@ typedef struct {
   uint64_t sum_product;   /* reference aggregate: sum of (int32)(measure*payload) as u64 (join.cu:96) */
   float time_build;       /* TimeKeeper fields (ms), join.cu:110-115 */
   float time_probe;
-  float time_extra;       /* table/filter fills */
-  float time_total;
+  float time_extra;       /* the clear of the table, filter and counters inside the join's
+                             window (the reference's timed memsets, bloom_join.cu:135-137) */
+  float time_total;       /* clear + build + probe */
   lj_probe_stats stats;
 } lj_join_result;
 
@@ -227,10 +228,17 @@ typedef struct {
   int64_t n_groups;        /* non-empty groups (may exceed groups_cap: then truncated) */
   int64_t n_rows;          /* rows aggregated */
   int64_t sum_total;
-  float time_query_ms;     /* runQuery window: fills + builds + probe + finalize */
+  float time_query_ms;     /* device time from the build launch to the end of the probe
+                              (builds + probe + finalize).  The reference's runQuery window
+                              (q33/0-lip.cu:216-278) also holds its allocations and memsets;
+                              here a query's tables come from the context's zero pool,
+                              cleared by a fill queued behind the query that last used them,
+                              so no clear runs inside this window.  lj_hash_join clears its
+                              tables inside its window and reports that as time_extra. */
   float time_build_ms;
   float time_probe_ms;
-  float reserved0;
+  int32_t status;          /* this query's own lj_status (batches and serving report one
+                              per query; lj_star_join: its return value) */
   lj_probe_stats stats;
 } lj_star_result;
 
@@ -262,6 +270,9 @@ lj_status lj_batch_create(lj_ctx* ctx, const lj_star_desc* descs, int32_t n, con
 /* Runs the batch; results[i] is filled like lj_star_join's (groups ascending g,
  * at most groups_cap of them); time_query_ms is the whole batch's time. */
 lj_status lj_batch_run(lj_batch* b, lj_star_result* results);
+/* (results[i].status holds query i's own status; the return value is the
+ * first failure.  A query that hits more groups than groups_cap fails with
+ * LJ_EUNSUPPORTED and its accumulators are cleared before the next run.) */
 void lj_batch_destroy(lj_batch* b);
 
 /* ---- serving ---------------------------------------------------------------------
@@ -300,7 +311,11 @@ typedef struct {
 } lj_serve_stats;
 
 lj_status lj_serve_start(lj_ctx* ctx, const lj_serve_cfg* cfg, lj_server** out);
-/* Queues a request (the descriptor is copied); *id identifies it for poll. */
+/* Queues a request (the descriptor is copied); *id identifies it for poll.
+ * The descriptor is checked here (lj_last_error(NULL) on the calling thread
+ * gives the message of a refusal), so a malformed request never joins a
+ * launch; a request that fails later (planning, out-of-domain keys, more
+ * groups than groups_cap) fails alone, with its own status. */
 lj_status lj_serve_submit(lj_server* srv, const lj_star_desc* q, uint64_t* id);
 /* Waits up to timeout_ms (0: do not wait, < 0: forever) for request id.
  * LJ_ETIMEOUT while it is not done.  When done, fills out like lj_star_join

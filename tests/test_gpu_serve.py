@@ -100,3 +100,60 @@ def test_serve_requires_resident_dims(ctx, serve_setup):
     with pytest.raises(L.LJError):
         srv.submit(q)
     srv.stop()
+
+
+def _oracle_bloom(b):
+    import oracle as O
+    if b is None:
+        return O.bloom_spec()
+    if b[0] == "mod":
+        return O.bloom_spec(O.BLOOM_MOD, b[1], 1)
+    return O.bloom_spec(O.BLOOM_MULT, b[1], b[2], layout=b[3])
+
+
+def test_serve_config5_shape_against_oracle(ctx):
+    """The serving configuration at its own shape: 8 predicate templates over
+    a resident 2^22-key dimension (PHT_2 + the autotuner's Bloom filter each),
+    requests of 2^18..2^26 rows sliced from a resident 2^26-row pool, Poisson
+    arrivals, a 200 GB budget.  Every served result (rows, int64 sum, Bloom
+    passes) must equal the oracle's single join on the same slice and
+    template."""
+    import oracle as O
+    seed = W.seed_of(5) + 100
+    n_pool = 1 << 26
+    _flush("serve config-5 shape: templates and pool")
+    cols, tmpls = W.serve_templates(ctx, seed)
+    pool = W.serve_pool(ctx, seed, n_pool)
+    torch.cuda.synchronize()
+    reqs = W.serve_requests(seed, 160, len(tmpls), n_pool)
+    descs = [W.serve_desc(pool, tmpls[t], o, n) for (t, o, n) in reqs]
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / 5000.0, len(descs)))
+    _flush("serve config-5 shape: Poisson arrivals at 5000 req/s")
+    import time
+    srv = ctx.serve(hbm_budget=200 << 30, max_batch=64, max_inflight=2)
+    ids = []
+    t0 = time.perf_counter()
+    for i, d in enumerate(descs):
+        while time.perf_counter() - t0 < arrivals[i]:
+            pass
+        ids.append(srv.submit(d))
+    out = [srv.poll(i, timeout_ms=120000) for i in ids]
+    st = srv.stats()
+    srv.stop()
+    assert st["completed"] == len(descs) and st["rejected"] == 0
+    _flush("serve config-5 shape: oracle")
+    keys = cols["keys"].cpu().numpy()
+    attr = cols["attr"].cpu().numpy()
+    fk = pool["fk"][:n_pool].cpu().numpy()
+    meas = pool["meas"][:n_pool].cpu().numpy()
+    n_dim = keys.size
+    specs = [O.DimSpec(keys, attr, n_dim, pred=list(W.single_join_pred(tm["threshold"])),
+                       bloom=_oracle_bloom(tm["bloom"])) for tm in tmpls]
+    for i, ((t, o, n), r) in enumerate(zip(reqs, out)):
+        assert r is not None and r["status"] == L._abi.OK, i
+        ref = O.single_join(specs[t], fk[o:o + n], meas[o:o + n], method=O.LIP, threads=16)
+        assert (r["n_rows"], r["sum_total"], r["stats"]["rows_bloom_pass"]) == \
+            (ref["n_match"], ref["sum_measure"], ref["n_bloom_pass"]), (i, t, o, n)
+    for tm in tmpls:
+        tm["dim"].release()
