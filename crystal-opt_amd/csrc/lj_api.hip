@@ -115,6 +115,64 @@ static const size_t kHdrBytes = 256;   // [0] groups hit, [1..16] statistics, [1
                                        // [25..27] kernel phase clocks
 static thread_local std::string g_create_err;
 
+static constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// A query's buffers in a batch or a serving launch, in three regions:
+//   zero  : gsum | gcount | gprod [ng each] | n_touched, done (kept zero
+//           between runs: the finalize clears what the query touched)
+//   plain : gpay [ng * 4] | touched [tcap] | statistics rows [rows * kStats]
+//   out   : result header | groups [tcap]
+struct QBufLayout {
+  size_t zero = 0, plain = 0, out = 0;
+};
+static QBufLayout qbuf_layout(int64_t ng, int64_t tcap, int64_t stats_rows) {
+  QBufLayout l;
+  l.zero = align256((size_t)ng * 8) * 3 + 256;
+  l.plain = align256((size_t)ng * kMaxDims * 4) + align256((size_t)tcap * 4) + align256((size_t)stats_rows * kStats * 8);
+  l.out = align256(kHdrBytes + sizeof(GroupOut) * (size_t)tcap);
+  return l;
+}
+static void qbuf_bind(ProbeArgs& a, int64_t ng, int64_t tcap, char* z, char* pl, char* od) {
+  const size_t g8 = align256((size_t)ng * 8);
+  a.gsum = reinterpret_cast<unsigned long long*>(z);
+  a.gcount = reinterpret_cast<unsigned long long*>(z + g8);
+  a.gprod = reinterpret_cast<unsigned long long*>(z + 2 * g8);
+  a.n_touched = reinterpret_cast<uint32_t*>(z + 3 * g8);
+  a.done = a.n_touched + 32;
+  a.gpay = reinterpret_cast<int32_t*>(pl);
+  a.touched = reinterpret_cast<uint32_t*>(pl + align256((size_t)ng * kMaxDims * 4));
+  a.stats = reinterpret_cast<long long*>(pl + align256((size_t)ng * kMaxDims * 4) + align256((size_t)tcap * 4));
+  a.touched_cap = (uint32_t)tcap;
+  a.out_hdr = reinterpret_cast<long long*>(od);
+  a.out = reinterpret_cast<GroupOut*>(od + kHdrBytes);
+  a.out_cap = tcap;
+}
+
+// ---------------------------------------------------- no-allocation scratch --
+// lj_star_join_noalloc runs on `scratch`, a file-scope device array: the
+// runtime gives every GPU its own copy when the library loads, and each GPU
+// holds it until the process ends.  Its regions are those of qbuf_layout for
+// kNaGroups groups, kNaListed listed groups and kNaRows statistics rows
+// (20.3 MB); the zero region is cleared once per device by lj_create and is
+// kept zero by the finalize (and by a clear after a list overflow).  Host
+// code never uses the array's name as an address: it uses the address that
+// hipGetSymbolAddress returned for the device (g_na_ptr).
+constexpr int64_t kNaGroups = 1 << 19;    // group space (Q3.3's 437,500 fits)
+constexpr int64_t kNaListed = 4096;       // groups a query may hit (the pinned result buffer's size)
+constexpr int64_t kNaRows = 2048;         // workgroups of one probe launch
+constexpr size_t kNaZeroBytes = align256((size_t)kNaGroups * 8) * 3 + 256;
+constexpr size_t kNaPlainBytes =
+    align256((size_t)kNaGroups * kMaxDims * 4) + align256((size_t)kNaListed * 4) + align256((size_t)kNaRows * kStats * 8);
+constexpr size_t kNaOutBytes = align256(kHdrBytes + sizeof(GroupOut) * (size_t)kNaListed);
+constexpr size_t kNaScratchBytes = kNaZeroBytes + kNaPlainBytes + kNaOutBytes;
+constexpr int kNaMaxDevices = 64;
+static_assert(kNaListed == kHostGroups, "the result goes through the context's pinned buffer");
+
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+static std::mutex g_na_mu;
+static char* g_na_ptr[kNaMaxDevices];   // scratch's address on each device (null: not set up)
+
 static lj_status fail(lj_ctx* c, lj_status s, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
 static lj_status fail(lj_ctx* c, lj_status s, const char* fmt, ...) {
   char buf[512];
@@ -364,6 +422,24 @@ extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* 
   c->probe_wg = env_int("LJ_PROBE_WG", 0);
   c->probe_vec = env_int("LJ_PROBE_VEC", 0);
   c->lds_policy = env_int("LJ_LDS_POLICY", -1);
+  // lj_star_join_noalloc's scratch on this device: its address, once per
+  // device (the device is current), and the one clear of its zero region
+  if (device < kNaMaxDevices) {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[device]) {
+      void* sp = nullptr;
+      const hipError_t e2 = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+      if (e2 != hipSuccess || !sp)
+        return bail(fail(c, LJ_EHIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e2)));
+      FillArgs f{};
+      f.r[0].ptr = static_cast<uint32_t*>(sp);
+      f.r[0].words = (int64_t)(kNaZeroBytes / 4);
+      f.n = 1;
+      if (launch_fill(f, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+        return bail(fail(c, LJ_EHIP, "clearing the no-allocation scratch failed"));
+      g_na_ptr[device] = static_cast<char*>(sp);
+    }
+  }
   *out = c;
   return LJ_OK;
 }
@@ -700,12 +776,14 @@ struct QueryOut {
   int64_t dim_oob[kMaxDims];
 };
 
-static lj_status occupancy_of(lj_ctx* c, const ProbeConfig& cfg, size_t lds_bytes, int* per_cu) {
+// Workgroups per CU of a probe configuration, cached per context (cache
+// false: a cached value is used, a new one is not stored).
+static lj_status occupancy_of(lj_ctx* c, const ProbeConfig& cfg, size_t lds_bytes, int* per_cu, bool cache = true) {
   const auto key = std::make_pair(cfg.wg * 1000 + cfg.vec * 100 + cfg.k0, lds_bytes);
   auto it = c->occupancy.find(key);
   if (it == c->occupancy.end()) {
     HIP_TRY(c, probe_occupancy(cfg, lds_bytes, per_cu));
-    c->occupancy[key] = *per_cu;
+    if (cache) c->occupancy[key] = *per_cu;
   } else {
     *per_cu = it->second;
   }
@@ -773,7 +851,8 @@ static lj_status acquire_dims(lj_ctx* c, const lj_star_desc* q, Plan& p, bool ze
 }
 
 // Workgroup shape, LDS-resident filters, stage-0 kernel kind, occupancy.
-static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, int wg, Plan& p) {
+static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, int wg, Plan& p,
+                            bool cache_occupancy = true) {
   const int nd = p.nd;
   const bool lip = q->method != LJ_METHOD_NAIVE;
   const bool adaptive = q->method == LJ_METHOD_LIP_ADAPTIVE;
@@ -857,7 +936,7 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   for (int d = 0; d < nd; d++)
     if (lip && kind_of(d) != cfg.k0) a.all_k0 = 0;
   p.lds_bytes = probe_lds_bytes(cfg, lds_words);
-  lj_status s = occupancy_of(c, cfg, p.lds_bytes, &p.per_cu);
+  lj_status s = occupancy_of(c, cfg, p.lds_bytes, &p.per_cu, cache_occupancy);
   if (s != LJ_OK) return s;
   a.measure = q->fact_measure;
   a.measure_sub = q->fact_measure_sub;
@@ -1059,12 +1138,13 @@ static void fill_stats(const QueryOut& qo, int nd, lj_probe_stats* ps) {
   for (int s = 0; s < LJ_MAX_DIMS; s++) ps->order_used[s] = s < nd ? (int32_t)((qo.stats[ST_ORDER] >> (8 * s)) & 0xff) : -1;
 }
 
-static void to_star_result(const lj_star_desc* q, const QueryOut& qo, lj_star_result* out) {
-  out->n_groups = (int64_t)qo.groups.size();
+// The caller's result from n groups in ascending g and the statistics.
+static void groups_to_result(const lj_star_desc* q, const GroupOut* groups, size_t n, lj_star_result* out) {
+  out->n_groups = (int64_t)n;
   out->n_rows = 0;
   out->sum_total = 0;
-  for (size_t i = 0; i < qo.groups.size(); i++) {
-    const GroupOut& g = qo.groups[i];
+  for (size_t i = 0; i < n; i++) {
+    const GroupOut& g = groups[i];
     out->n_rows += (int64_t)g.count;
     out->sum_total += g.sum;
     if ((int64_t)i < out->groups_cap && out->groups) {
@@ -1077,6 +1157,10 @@ static void to_star_result(const lj_star_desc* q, const QueryOut& qo, lj_star_re
       o.reserved0 = 0;
     }
   }
+}
+
+static void to_star_result(const lj_star_desc* q, const QueryOut& qo, lj_star_result* out) {
+  groups_to_result(q, qo.groups.data(), qo.groups.size(), out);
   out->time_query_ms = qo.t_total;
   out->time_build_ms = qo.t_fill + qo.t_build;
   out->time_probe_ms = qo.t_probe;
@@ -1093,6 +1177,73 @@ extern "C" lj_status lj_star_join(lj_ctx* c, const lj_star_desc* q, lj_star_resu
     return s;
   }
   to_star_result(q, qo, out);
+  out->status = s;
+  return s;
+}
+
+// lj_star_join over resident dimensions with no allocation: the probe's
+// buffers are this device's `scratch` (its address from lj_create), the
+// result comes through the context's pinned buffer, the groups are sorted in
+// place there.  Nothing here allocates device or host memory.
+extern "C" lj_status lj_star_join_noalloc(lj_ctx* c, const lj_star_desc* q, lj_star_result* out) {
+  if (!c || !q || !out) return fail(c, LJ_EINVAL, "NULL argument");
+  Plan p;
+  lj_status s = validate_query(c, q, p);
+  if (s != LJ_OK) return s;
+  for (int d = 0; d < q->n_dims; d++)
+    if (!q->resident[d])
+      return fail(c, LJ_EINVAL, "lj_star_join_noalloc needs resident dimensions (dimension %d is not)", d);
+  if (q->n_groups > kNaGroups)
+    return fail(c, LJ_EINVAL, "n_groups %lld exceeds the no-allocation limit of %lld", (long long)q->n_groups,
+                (long long)kNaGroups);
+  char* base = c->device < kNaMaxDevices ? g_na_ptr[c->device] : nullptr;
+  if (!base) return fail(c, LJ_ESTATE, "no scratch was set up for device %d", c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
+  if ((s = acquire_dims(c, q, p)) != LJ_OK) return s;  // resident dimensions only: nothing allocated
+  if ((s = plan_probe(c, q, 0, 0, p, false)) != LJ_OK) return s;
+  const ProbeConfig& cfg = p.cfg;
+  ProbeArgs& a = p.a;
+  qbuf_bind(a, kNaGroups, kNaListed, base, base + kNaZeroBytes, base + kNaZeroBytes + kNaPlainBytes);
+  a.host_hdr = static_cast<long long*>(c->host_buf_dev);
+  a.host_out = reinterpret_cast<GroupOut*>(static_cast<char*>(c->host_buf_dev) + kHdrBytes);
+  a.host_groups = kNaListed;
+  const int64_t tile = (int64_t)cfg.vec * kSegRows * (cfg.wg / 64);
+  const int64_t need = std::max<int64_t>(1, (q->n_fact + tile - 1) / tile);
+  const int64_t grid = std::min<int64_t>({(int64_t)c->n_cu * p.per_cu, need, kNaRows});
+  HIP_TRY(c, hipEventRecord(c->ev[3], c->stream));
+  HIP_TRY(c, launch_probe(cfg, a, (int)grid, p.lds_bytes, c->stream));
+  HIP_TRY(c, hipEventRecord(c->ev[kEvDone], c->stream));
+  HIP_TRY(c, spin_wait(c->ev[kEvDone]));
+  const long long* hdr = static_cast<const long long*>(c->host_buf);
+  GroupOut* hg = reinterpret_cast<GroupOut*>(static_cast<char*>(c->host_buf) + kHdrBytes);
+  const int64_t n_touched = hdr[0];
+  if (n_touched > kNaListed) {
+    // the finalize zeroed only the listed groups: clear the whole region, and
+    // wait for the clear, so that the next call (on any stream) finds it zero
+    FillArgs f{};
+    f.r[0].ptr = reinterpret_cast<uint32_t*>(base);
+    f.r[0].words = (int64_t)(kNaZeroBytes / 4);
+    f.n = 1;
+    HIP_TRY(c, launch_fill(f, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return fail(c, LJ_EUNSUPPORTED, "%lld groups were hit; the no-allocation path lists at most %lld",
+                (long long)n_touched, (long long)kNaListed);
+  }
+  std::sort(hg, hg + n_touched, [](const GroupOut& x, const GroupOut& y) { return x.g < y.g; });
+  groups_to_result(q, hg, (size_t)n_touched, out);
+  QueryOut qo;  // statistics and timings only (its group vector stays empty)
+  memcpy(qo.stats, hdr + 1, sizeof qo.stats);
+  HIP_TRY(c, hipEventElapsedTime(&qo.t_probe, c->ev[3], c->ev[kEvDone]));
+  out->time_query_ms = qo.t_probe;
+  out->time_build_ms = 0.f;
+  out->time_probe_ms = qo.t_probe;
+  fill_stats(qo, q->n_dims, &out->stats);
+  int64_t build_oob = 0;
+  for (int d = 0; d < q->n_dims; d++) build_oob += p.dims[d]->n_oob;
+  s = LJ_OK;
+  if (build_oob > 0 || qo.stats[ST_OOB] > 0)
+    s = fail(c, LJ_EDOMAIN, "%lld dimension keys and %lld fact keys/groups fell outside their hash domains",
+             (long long)build_oob, (long long)qo.stats[ST_OOB]);
   out->status = s;
   return s;
 }
@@ -1202,38 +1353,6 @@ struct BatchKind {
   ProbeArgs* d_qs = nullptr;
   uint32_t* d_wgb = nullptr;
 };
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-// A query's buffers in a batch or a serving launch, in three regions:
-//   zero  : gsum | gcount | gprod [ng each] | n_touched, done (kept zero
-//           between runs: the finalize clears what the query touched)
-//   plain : gpay [ng * 4] | touched [tcap] | statistics rows [rows * kStats]
-//   out   : result header | groups [tcap]
-struct QBufLayout {
-  size_t zero = 0, plain = 0, out = 0;
-};
-QBufLayout qbuf_layout(int64_t ng, int64_t tcap, int64_t stats_rows) {
-  QBufLayout l;
-  l.zero = align256((size_t)ng * 8) * 3 + 256;
-  l.plain = align256((size_t)ng * kMaxDims * 4) + align256((size_t)tcap * 4) + align256((size_t)stats_rows * kStats * 8);
-  l.out = align256(kHdrBytes + sizeof(GroupOut) * (size_t)tcap);
-  return l;
-}
-void qbuf_bind(ProbeArgs& a, int64_t ng, int64_t tcap, char* z, char* pl, char* od) {
-  const size_t g8 = align256((size_t)ng * 8);
-  a.gsum = reinterpret_cast<unsigned long long*>(z);
-  a.gcount = reinterpret_cast<unsigned long long*>(z + g8);
-  a.gprod = reinterpret_cast<unsigned long long*>(z + 2 * g8);
-  a.n_touched = reinterpret_cast<uint32_t*>(z + 3 * g8);
-  a.done = a.n_touched + 32;
-  a.gpay = reinterpret_cast<int32_t*>(pl);
-  a.touched = reinterpret_cast<uint32_t*>(pl + align256((size_t)ng * kMaxDims * 4));
-  a.stats = reinterpret_cast<long long*>(pl + align256((size_t)ng * kMaxDims * 4) + align256((size_t)tcap * 4));
-  a.touched_cap = (uint32_t)tcap;
-  a.out_hdr = reinterpret_cast<long long*>(od);
-  a.out = reinterpret_cast<GroupOut*>(od + kHdrBytes);
-  a.out_cap = tcap;
-}
 }  // namespace
 
 struct lj_batch {

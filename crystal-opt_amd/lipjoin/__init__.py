@@ -211,7 +211,12 @@ class Context:
         r = self._raw_res
         return _star_result_dict(r, self._gbuf, self._raw_nd, r.groups_cap)
 
-    def star_join(self, desc, groups_cap=4096, allow_domain=False):
+    def star_join_noalloc(self, desc, groups_cap=4096, allow_domain=False):
+        """lj_star_join_noalloc: the same result as star_join, over resident
+        dimensions, with the library allocating nothing."""
+        return self.star_join(desc, groups_cap, allow_domain, _entry="lj_star_join_noalloc")
+
+    def star_join(self, desc, groups_cap=4096, allow_domain=False, _entry="lj_star_join"):
         # the host result buffer is kept and reused across calls
         if getattr(self, "_gbuf_cap", 0) < groups_cap:
             self._gbuf = (_abi.Group * groups_cap)()
@@ -219,7 +224,7 @@ class Context:
         arr = self._gbuf
         r = _abi.StarResult()
         r.groups, r.groups_cap = C.cast(arr, C.POINTER(_abi.Group)), groups_cap
-        st = self._check(lib().lj_star_join(self.h, C.byref(desc), C.byref(r)),
+        st = self._check(getattr(lib(), _entry)(self.h, C.byref(desc), C.byref(r)),
                          allow=(_abi.EDOMAIN,) if allow_domain else ())
         n = min(r.n_groups, groups_cap)
         groups = [(arr[i].g, tuple(arr[i].payload[: desc.n_dims]), arr[i].sum, arr[i].count,

@@ -147,6 +147,7 @@ SIGNATURES = {
     "lj_dim_release": (None, [C.c_void_p, C.c_void_p]),
     "lj_hash_join": (C.c_int, [C.c_void_p, C.POINTER(JoinDesc), C.POINTER(JoinResult)]),
     "lj_star_join": (C.c_int, [C.c_void_p, C.POINTER(StarDesc), C.POINTER(StarResult)]),
+    "lj_star_join_noalloc": (C.c_int, [C.c_void_p, C.POINTER(StarDesc), C.POINTER(StarResult)]),
     "lj_batch_create": (C.c_int, [C.c_void_p, C.POINTER(StarDesc), C.c_int32, C.POINTER(BatchOpts),
                                   C.POINTER(C.c_void_p)]),
     "lj_batch_run": (C.c_int, [C.c_void_p, C.POINTER(StarResult)]),

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define LJ_ABI_VERSION 2
+#define LJ_ABI_VERSION 3
 #define LJ_MAX_DIMS 4
 #define LJ_MAX_PRED_STEPS 4
 
@@ -243,6 +243,21 @@ typedef struct {
 } lj_star_result;
 
 lj_status lj_star_join(lj_ctx* ctx, const lj_star_desc* desc, lj_star_result* out);
+
+/* lj_star_join_noalloc -- lj_star_join that allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Every dimension must be resident
+ * (lj_dim_build); the caller passes the fact columns and the result struct.
+ * The probe's buffers (group accumulators, statistics rows, the group list,
+ * the result region) are a fixed device array of 20.3 MB that each GPU holds
+ * from the library's load until the process ends (set up by lj_create).
+ * Limits: n_groups <= LJ_NOALLOC_MAX_GROUPS, at most LJ_NOALLOC_MAX_LISTED
+ * groups hit (else LJ_EINVAL / LJ_EUNSUPPORTED, and the array is cleared).
+ * All calls on one GPU share the array: calls that overlap on one GPU race. */
+#define LJ_NOALLOC_MAX_GROUPS (1 << 19)
+#define LJ_NOALLOC_MAX_LISTED 4096
+lj_status lj_star_join_noalloc(lj_ctx* ctx, const lj_star_desc* desc, lj_star_result* out);
 
 /* ---- batches of queries ---------------------------------------------------------
  * n independent star joins (any mix of sizes, predicates and Bloom filters)
