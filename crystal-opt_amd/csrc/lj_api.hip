@@ -1900,7 +1900,7 @@ struct ServeDone {
 };
 
 struct ServeSlot {
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // one of the server's streams
   hipEvent_t e0 = nullptr, e1 = nullptr;
   char* args_dev = nullptr;   // ProbeArgs [max_batch] | wg_begin [max_batch + S0_NKINDS]
   char* args_host = nullptr;  // pinned staging of args_dev
@@ -1932,6 +1932,7 @@ struct lj_server {
   bool stopping = false;
   lj_serve_stats stats{};
   std::vector<ServeSlot> slots;
+  std::vector<hipStream_t> streams;
   std::multimap<int64_t, ReqBuf*> free_bufs;  // idle request buffers by group class
   std::vector<ReqBuf*> all_bufs;
   int64_t stats_rows = 0;
@@ -2253,14 +2254,14 @@ static void serve_loop(lj_server* S) {
 
 static void server_free(lj_server* S) {
   lj_ctx* c = S->c;
+  for (hipStream_t st : S->streams) (void)hipStreamSynchronize(st);
   for (ServeSlot& sl : S->slots) {
-    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.e0) (void)hipEventDestroy(sl.e0);
     if (sl.e1) (void)hipEventDestroy(sl.e1);
     dev_release(c, sl.args_dev);
     if (sl.args_host) (void)hipHostFree(sl.args_host);
-    if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
+  for (hipStream_t st : S->streams) (void)hipStreamDestroy(st);
   for (ReqBuf* b : S->all_bufs) reqbuf_free(c, b);
   delete S;
 }
@@ -2279,16 +2280,27 @@ extern "C" lj_status lj_serve_start(lj_ctx* c, const lj_serve_cfg* cfg, lj_serve
     delete S;
     return fail(c, LJ_EINVAL, "max_batch <= 4096 and max_inflight <= 8");
   }
+  if (S->cfg.streams <= 0) S->cfg.streams = 1;
+  S->cfg.streams = std::min(S->cfg.streams, S->cfg.max_inflight);
   S->stats_rows = (int64_t)c->n_cu * kServeWgPerCu;
+  for (int i = 0; i < S->cfg.streams; i++) {
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      server_free(S);
+      return fail(c, LJ_EHIP, "hipStreamCreate failed");
+    }
+    S->streams.push_back(st);
+  }
   S->slots.resize((size_t)S->cfg.max_inflight);
   const size_t nb = (size_t)S->cfg.max_batch;
-  for (ServeSlot& sl : S->slots) {
+  for (size_t i = 0; i < S->slots.size(); i++) {
+    ServeSlot& sl = S->slots[i];
+    sl.stream = S->streams[i % S->streams.size()];
     sl.wgb_off = align256(sizeof(ProbeArgs) * nb);
     sl.args_bytes = sl.wgb_off + align256(sizeof(uint32_t) * (nb + S0_NKINDS));
     void* d = nullptr;
     lj_status s = LJ_OK;
-    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.e0, hipEventDefault) != hipSuccess ||
+    if (hipEventCreateWithFlags(&sl.e0, hipEventDefault) != hipSuccess ||
         hipEventCreateWithFlags(&sl.e1, hipEventDefault) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&sl.args_host), sl.args_bytes, hipHostMallocDefault) != hipSuccess)
       s = fail(c, LJ_EHIP, "serving slot setup failed");

@@ -119,7 +119,8 @@ class BloomChoice(C.Structure):
 
 class ServeCfg(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
-                ("groups_cap", C.c_int64), ("reserved", C.c_int64 * 2)]
+                ("groups_cap", C.c_int64), ("streams", C.c_int32), ("reserved0", C.c_int32),
+                ("reserved1", C.c_int64)]
 
 
 class ServeStats(C.Structure):

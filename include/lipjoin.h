@@ -299,8 +299,9 @@ void lj_batch_destroy(lj_batch* b);
  * Continuous batching: requests that arrive while launches are in flight
  * wait; whenever one of max_inflight launch slots is free, every waiting
  * request that the HBM budget admits (up to max_batch) goes into ONE launch
- * (the batched probe, on the slot's own stream, so launches overlap on the
- * device).  The budget bounds the per-request device memory in flight
+ * (the batched probe, whose queries' workgroups overlap on the device; the
+ * launch queues behind the one in flight, or runs beside it with
+ * cfg.streams > 1).  The budget bounds the per-request device memory in flight
  * (statistics rows, group accumulators, result regions).  Results are
  * identical to lj_star_join of the same query.  No reference counterpart:
  * the reference runs one query per process invocation. */
@@ -311,7 +312,12 @@ typedef struct {
   int32_t max_batch;       /* requests per launch; 0: 64 */
   int32_t max_inflight;    /* launches in flight, each on its own stream; 0: 2 */
   int64_t groups_cap;      /* result groups kept per request; 0: 64 */
-  int64_t reserved[2];
+  int32_t streams;         /* streams the launch slots use: 0 or 1 = one stream, so a new
+                              launch queues behind the one in flight (no two launches
+                              compete for the device's caches); k > 1: slot i runs on
+                              stream i % k and launches overlap on the device */
+  int32_t reserved0;
+  int64_t reserved1;
 } lj_serve_cfg;
 
 typedef struct {

@@ -55,7 +55,7 @@ def test_serve_same_alone_and_in_mix(ctx, serve_setup):
 def test_serve_concurrent_submitters(ctx, serve_setup):
     cols, tmpls, pool, reqs, descs, alone = serve_setup
     _flush("serve threads")
-    srv = ctx.serve(max_batch=16, max_inflight=3)
+    srv = ctx.serve(max_batch=16, max_inflight=3, streams=3)   # launches overlap on the device
     results = {}
 
     def client(k):
