@@ -488,9 +488,13 @@ static lj_status check_dim_desc(lj_ctx* c, const lj_dim_desc* d) {
       return fail(c, LJ_EINVAL, "predicate step %d has an unknown operator", i);
   }
   if (d->n_rows > 0 && d->pred.n_steps > 0 && !d->attr) return fail(c, LJ_EINVAL, "predicate needs the attr column");
-  if (d->n_rows > 0 && !d->payload && !d->attr) return fail(c, LJ_EINVAL, "PHT_2 needs a payload or attr column");
-  if (d->ht_kind != LJ_HT_PHT2) return fail(c, LJ_EUNSUPPORTED, "ht_kind %d not supported", d->ht_kind);
+  if (d->ht_kind != LJ_HT_PHT2 && d->ht_kind != LJ_HT_PHT1 && d->ht_kind != LJ_HT_TEXTBOOK)
+    return fail(c, LJ_EINVAL, "ht_kind %d unknown", d->ht_kind);
+  if (d->n_rows > 0 && d->ht_kind != LJ_HT_PHT1 && !d->payload && !d->attr)
+    return fail(c, LJ_EINVAL, "the table needs a payload or attr column");
   if (d->ht_len < 1 || d->ht_len > (1ll << 30)) return fail(c, LJ_EINVAL, "ht_len %lld out of range [1, 2^30]", (long long)d->ht_len);
+  if (d->ht_kind == LJ_HT_TEXTBOOK && (d->ht_len & (d->ht_len - 1)))
+    return fail(c, LJ_EINVAL, "a textbook table needs a power-of-two ht_len (got %lld)", (long long)d->ht_len);
   if (d->bloom_hash == LJ_BLOOM_MOD) {
     if (d->bloom_k != 1) return fail(c, LJ_EINVAL, "MOD Bloom hash supports k = 1 only");
     if (d->bloom_bits < 32 || d->bloom_bits > (1ll << 31)) return fail(c, LJ_EINVAL, "bloom_bits %lld out of range", (long long)d->bloom_bits);
@@ -509,7 +513,7 @@ static lj_status check_dim_desc(lj_ctx* c, const lj_dim_desc* d) {
 static void dim_hashes(lj_dim* D) {
   const lj_dim_desc& d = D->desc;
   D->ht_len = make_fastmod((uint32_t)d.ht_len);
-  D->ht_words = 2 * d.ht_len;
+  D->ht_words = d.ht_kind == LJ_HT_PHT1 ? d.ht_len : 2 * d.ht_len;
   BloomHash& b = D->bh;
   b = BloomHash{};
   b.kind = BLOOM_NONE;
@@ -621,6 +625,7 @@ static void set_build_dim(const lj_dim* D, BuildDim& bd) {
   bd.ht = D->ht;
   bd.ht_len = D->ht_len;
   bd.keys_min = D->desc.keys_min;
+  bd.ht_kind = D->desc.ht_kind;
   bd.bloom = D->bloom;
   bd.bh = D->bh;
   bd.counters = D->counters;
@@ -874,7 +879,10 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
     P.bh = lip ? D->bh : BloomHash{};
     if (!lip) P.bh.kind = BLOOM_NONE;
     P.ht_len = D->ht_len;
-    P.keys_min = D->desc.keys_min;
+    P.ht_kind = D->desc.ht_kind;
+    // the fact keys outside the table's domain: below keys_min (perfect
+    // hash), the excluded key INT32_MIN (textbook)
+    P.keys_min = D->desc.ht_kind == LJ_HT_TEXTBOOK ? kTbExcludedKey + 1 : D->desc.keys_min;
     P.bloom_lds = -1;
     P.bloom_words = (int32_t)D->bloom_words;
     P.gmult = q->group_mult[d];

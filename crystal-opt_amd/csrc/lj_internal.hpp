@@ -48,9 +48,9 @@ struct BuildDim {
   int64_t n;
   PredProgram pred;
   int32_t* ht;
-  FastMod ht_len;
+  FastMod ht_len;          // textbook: pow2_mask = slots - 1
   int32_t keys_min;
-  int32_t pad0;
+  int32_t ht_kind;         // HtKind
   uint32_t* bloom;         // nullptr: no filter
   BloomHash bh;
   unsigned long long* counters;  // kCounterLanes x {selected rows, keys outside the domain}
@@ -66,13 +66,13 @@ constexpr int kCounterWords = 2 * kCounterLanes + 2;  // + popcount scratch
 struct ProbeDim {
   const int32_t* fk;      // fact FK column of this dimension
   const uint32_t* bloom;  // global Bloom words (nullptr: no filter)
-  const int32_t* ht;      // PHT_2 pairs
+  const int32_t* ht;      // the hash table (PHT_2 / textbook pairs, PHT_1 keys)
   BloomHash bh;
-  FastMod ht_len;
-  int32_t keys_min;       // PHT's HASH Z argument
+  FastMod ht_len;          // textbook: pow2_mask = slots - 1
+  int32_t keys_min;       // PHT's HASH Z argument (textbook: INT32_MIN + 1, its domain's bound)
   int32_t bloom_lds;      // word offset of the staged copy in LDS, -1 = global
   int32_t bloom_words;
-  int32_t pad0;
+  int32_t ht_kind;        // HtKind
   int64_t gmult;
 };
 

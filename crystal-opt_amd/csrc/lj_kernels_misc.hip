@@ -50,7 +50,8 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
 
 // --------------------------------------------------------------- build ----
 // Predicate, Bloom insert, PHT_2 insert (CAS of the key, then the
-// unconditional value store) -- build_kernel of src/ops/bloom_join.cu:43-65 and
+// unconditional value store; PHT_1: the CAS alone, join.cuh:191-258; or the
+// textbook table's insert) -- build_kernel of src/ops/bloom_join.cu:43-65 and
 // build_hashtable_{s,c,d} of src/ssb-jo/q33/0-lip.cu:114-203.  Every dimension
 // of the query in one launch: each dimension owns a range of workgroups (all
 // dimensions build at once), a thread takes 4 consecutive rows (1 in a
@@ -62,6 +63,25 @@ constexpr int kBuildRows = 4;
 constexpr int64_t kBuildSmall = 1 << 16;
 
 __host__ __device__ inline int build_rows_per_thread(int64_t rows) { return rows < kBuildSmall ? 1 : kBuildRows; }
+
+// Textbook insert: claim the first slot from the key's home slot on that is
+// empty or already holds the key (compare-and-swap of the stored key), then
+// store the value (a repeated key keeps its slot; the last value stays).
+// False when the key is the excluded one or the table is full.
+__device__ __forceinline__ bool tb_insert(int32_t* ht, uint32_t mask, int32_t key, int32_t val) {
+  if (key == kTbExcludedKey) return false;
+  const int32_t enc = (int32_t)tb_encode(key);
+  uint32_t s = tb_home(key, mask);
+  for (uint32_t i = 0; i <= mask; i++) {
+    const int32_t old = atomicCAS(&ht[2 * (int64_t)s], 0, enc);
+    if (old == 0 || old == enc) {
+      ht[2 * (int64_t)s + 1] = val;
+      return true;
+    }
+    s = (s + 1) & mask;
+  }
+  return false;
+}
 
 // Workgroup `blk` of dimension B: rows [blk * 256 * rpt, (blk + 1) * 256 * rpt).
 __device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
@@ -80,12 +100,18 @@ __device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
       if (sel) {
         nsel++;
         if (B.bloom) bloom_insert(B.bh, B.bloom, key);
-        const int64_t s = pht_slot(key, B.ht_len, B.keys_min);
-        if (s < 0) {
-          noob++;
+        if (B.ht_kind == HT_TEXTBOOK) {
+          if (!tb_insert(B.ht, B.ht_len.pow2_mask, key, B.payload[r])) noob++;
         } else {
-          atomicCAS(&B.ht[2 * s], 0, key);
-          B.ht[2 * s + 1] = B.payload[r];
+          const int64_t s = pht_slot(key, B.ht_len, B.keys_min);
+          if (s < 0) {
+            noob++;
+          } else if (B.ht_kind == HT_PHT1) {
+            atomicCAS(&B.ht[s], 0, key);
+          } else {
+            atomicCAS(&B.ht[2 * s], 0, key);
+            B.ht[2 * s + 1] = B.payload[r];
+          }
         }
       }
     }

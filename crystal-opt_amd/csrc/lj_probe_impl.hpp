@@ -105,14 +105,46 @@ __device__ __forceinline__ bool bloom_probe_rt(const ProbeDim& D, const uint32_t
   return bloom_test(D.bh, (GlobalWords)D.bloom, key);
 }
 
-// pht2_probe (join.hpp) with the slot read as a global load
-__device__ __forceinline__ bool pht_probe(const ProbeDim& D, int32_t key, int32_t& pay) {
-  const int64_t s = pht_slot(key, D.ht_len, D.keys_min);
-  if (s < 0) return false;
-  const uint64_t slot = gl(reinterpret_cast<const uint64_t*>(D.ht))[s];
-  if (slot == 0) return false;
-  pay = (int32_t)(uint32_t)(slot >> 32);
+// ---- hash-table lookups (the kind is uniform: a scalar branch) -------------
+// PHT_2 (join.cuh:108-189): one 64-bit load of the (key, value) pair, a hit
+// is a nonzero pair, no key compare.  PHT_1 (join.cuh:29-106): one 32-bit
+// load, a hit is a nonzero key, no payload (0).  Textbook: the pair of the
+// home slot, then linear probing until the key or an empty slot.
+// ht_first_slot: the slot to load first (-1: the key is outside the domain).
+__device__ __forceinline__ int64_t ht_first_slot(int32_t kind, const FastMod& len, int32_t kmin, int32_t key) {
+  if (kind == HT_TEXTBOOK) return key == kTbExcludedKey ? -1 : (int64_t)tb_home(key, len.pow2_mask);
+  return pht_slot(key, len, kmin);
+}
+__device__ __forceinline__ uint64_t ht_load(int32_t kind, const int32_t* ht, int64_t s) {
+  if (kind == HT_PHT1) return (uint64_t)(uint32_t)gl(ht)[s];
+  return gl(reinterpret_cast<const uint64_t*>(ht))[s];
+}
+// Hit and payload from `raw`, the first load (slot s >= 0).
+__device__ __forceinline__ bool ht_resolve(int32_t kind, const int32_t* ht, const FastMod& len, int32_t key,
+                                           int64_t s, uint64_t raw, int32_t& pay) {
+  if (kind == HT_TEXTBOOK) {
+    const uint32_t enc = tb_encode(key), mask = len.pow2_mask;
+    for (uint32_t i = 0; i <= mask; i++) {
+      const uint32_t k = (uint32_t)raw;
+      if (k == enc) {
+        pay = (int32_t)(uint32_t)(raw >> 32);
+        return true;
+      }
+      if (k == 0) return false;
+      s = (s + 1) & mask;
+      raw = gl(reinterpret_cast<const uint64_t*>(ht))[s];
+    }
+    return false;
+  }
+  if (raw == 0) return false;
+  pay = (int32_t)(uint32_t)(raw >> 32);
   return true;
+}
+
+__device__ __forceinline__ bool pht_probe(const ProbeDim& D, int32_t key, int32_t& pay) {
+  const int64_t s = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, key);
+  if (s < 0) return false;
+  return ht_resolve(D.ht_kind, D.ht, D.ht_len, key, s, ht_load(D.ht_kind, D.ht, s), pay);
 }
 
 // A filter probe outside stage 0 (later stages, the adaptive sample).  When
@@ -154,6 +186,7 @@ struct S0 {
   FastMod fm;
   int32_t kmin;
   uint32_t mask;
+  int32_t ht_kind;
 };
 
 template <int K0>
@@ -195,17 +228,36 @@ __device__ __forceinline__ uint32_t s0_probe4(const S0& s, const v4i32& k, uint3
       pay[j] = 0;
     }
   } else if constexpr (K0 == S0_PHT) {
-    uint64_t slot[4];
+    if (s.ht_kind == HT_PHT2) {
+      uint64_t slot[4];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int64_t sl = pht_slot(kk[j], s.fm, s.kmin);
-      slot[j] = gl(reinterpret_cast<const uint64_t*>(s.ht))[sl < 0 ? 0 : sl];
-      ok |= (uint32_t)(sl >= 0) << j;
-    }
+      for (int j = 0; j < 4; j++) {
+        const int64_t sl = pht_slot(kk[j], s.fm, s.kmin);
+        slot[j] = gl(reinterpret_cast<const uint64_t*>(s.ht))[sl < 0 ? 0 : sl];
+        ok |= (uint32_t)(sl >= 0) << j;
+      }
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      ok &= ~((uint32_t)(slot[j] == 0) << j);
-      pay[j] = (int32_t)(uint32_t)(slot[j] >> 32);
+      for (int j = 0; j < 4; j++) {
+        ok &= ~((uint32_t)(slot[j] == 0) << j);
+        pay[j] = (int32_t)(uint32_t)(slot[j] >> 32);
+      }
+    } else {
+      // PHT_1 / textbook: the first loads of the 4 rows together, then each
+      // row's resolution (a textbook row may probe further slots)
+      int64_t sl[4];
+      uint64_t raw[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        sl[j] = ht_first_slot(s.ht_kind, s.fm, s.kmin, kk[j]);
+        raw[j] = ht_load(s.ht_kind, s.ht, sl[j] < 0 ? 0 : sl[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        int32_t p = 0;
+        const bool hit = ((valid4 >> j) & 1u) && sl[j] >= 0 && ht_resolve(s.ht_kind, s.ht, s.fm, kk[j], sl[j], raw[j], p);
+        ok |= (uint32_t)hit << j;
+        pay[j] = p;
+      }
     }
   } else {
 #pragma unroll
@@ -221,7 +273,8 @@ __device__ __forceinline__ uint32_t s0_probe4(const S0& s, const v4i32& k, uint3
 template <int K0>
 __device__ __forceinline__ bool s0_probe(const S0& s, int32_t key, int32_t& pay) {
   if constexpr (K0 == S0_PHT) {
-    return pht2_probe(s.ht, s.fm, s.kmin, key, pay);
+    const int64_t sl = ht_first_slot(s.ht_kind, s.fm, s.kmin, key);
+    return sl >= 0 && ht_resolve(s.ht_kind, s.ht, s.fm, key, sl, ht_load(s.ht_kind, s.ht, sl), pay);
   } else if constexpr (K0 == S0_NONE) {
     return true;
   } else if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) {
@@ -435,8 +488,8 @@ __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int 
     m = r < 0 ? x1 : gl(a.measure)[row];
     if (LIP) {
       const ProbeDim& D = a.dim[0];
-      const int64_t sl = pht_slot(x0, D.ht_len, D.keys_min);
-      if (sl >= 0) ht = gl(reinterpret_cast<const uint64_t*>(D.ht + 2 * sl))[0];
+      const int64_t sl = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, x0);
+      if (sl >= 0) ht = ht_load(D.ht_kind, D.ht, sl);
     }
   }
   w.p_ht = ht;
@@ -499,9 +552,18 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
     const unsigned long long bloom_pass = __ballot(alive);
     st.v[ST_BLOOM_PASS] += __popcll(bloom_pass);
     if (nd == 1) {
-      // the slot gathered by issue_batch (pht2_probe: a nonzero slot is a hit)
-      alive = alive && w.p_ht != 0;
-      pay.set(0, (int32_t)(uint32_t)(w.p_ht >> 32));
+      // the slot gathered by issue_batch (a perfect hash: a nonzero slot is a
+      // hit; a textbook table goes on probing from it)
+      const ProbeDim& D = a.dim[0];
+      int32_t p = 0;
+      if (D.ht_kind == HT_TEXTBOOK) {
+        const int64_t sl = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, w.p_x0);
+        alive = alive && sl >= 0 && ht_resolve(D.ht_kind, D.ht, D.ht_len, w.p_x0, sl, w.p_ht, p);
+      } else {
+        alive = alive && w.p_ht != 0;
+        p = (int32_t)(uint32_t)(w.p_ht >> 32);
+      }
+      pay.set(0, p);
       m = w.p_m;
     } else if (bloom_pass != 0) {
       // After the Bloom chain every key is known: the PHT_2 probes of all
@@ -513,22 +575,31 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
       // (every slot is loaded, the one of a dimension past nd from dimension
       // 0's table: no branch on nd between the loads either)
       uint64_t slot[kMaxDims];
+      int64_t sls[kMaxDims];
       bool hit = alive;
 #pragma unroll
       for (int t = 0; t < kMaxDims; t++) {
         const bool on = t < nd;
         const ProbeDim& D = a.dim[on ? t : 0];
-        const int64_t sl = pht_slot(key.get(t), D.ht_len, D.keys_min);
+        const int64_t sl = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, key.get(t));
         hit = hit && (!on || sl >= 0);
-        slot[t] = gl(reinterpret_cast<const uint64_t*>(D.ht))[alive && on && sl >= 0 ? sl : 0];
+        sls[t] = sl;
+        slot[t] = ht_load(D.ht_kind, D.ht, alive && on && sl >= 0 ? sl : 0);
       }
       m = gl(a.measure)[alive ? (uint32_t)row : 0u];
       if (a.measure_sub) m = (int32_t)((uint32_t)m - (uint32_t)gl(a.measure_sub)[alive ? (uint32_t)row : 0u]);
 #pragma unroll
       for (int t = 0; t < kMaxDims; t++) {
         if (t >= nd) continue;
-        hit = hit && slot[t] != 0;
-        pay.set(t, (int32_t)(uint32_t)(slot[t] >> 32));
+        const ProbeDim& D = a.dim[t];
+        int32_t p = 0;
+        if (D.ht_kind == HT_TEXTBOOK) {
+          if (hit) hit = ht_resolve(D.ht_kind, D.ht, D.ht_len, key.get(t), sls[t], slot[t], p);
+        } else {
+          hit = hit && slot[t] != 0;
+          p = (int32_t)(uint32_t)(slot[t] >> 32);
+        }
+        pay.set(t, p);
       }
       alive = hit;
     }
@@ -1194,6 +1265,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   s0.fm = D0.ht_len;
   s0.kmin = D0.keys_min;
   s0.mask = D0.bh.mod.pow2_mask;
+  s0.ht_kind = D0.ht_kind;
   const int32_t* eager_col = w.nd > 1 ? a.dim[w.ld1].fk : a.measure;
 
   TileCtx tc;

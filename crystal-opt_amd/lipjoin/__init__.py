@@ -16,8 +16,8 @@ from . import _abi
 from ._abi import (  # noqa: F401  (re-exported constants)
     BLOOM_BLOCKED, BLOOM_MOD, BLOOM_MULT, BLOOM_NONE, BLOOM_STANDARD, CMP_EQ, CMP_GT, CMP_GTE,
     CMP_LT, CMP_LTE, COMB_AND, COMB_OR, COMB_SET, GEN_DATEKEY, GEN_DATEKEY_SEQ, GEN_SEQUENCE,
-    GEN_UNIFORM, GEN_YEAR_OF_DAY, METHOD_LIP, METHOD_LIP_ADAPTIVE, METHOD_NAIVE, MAX_DIMS,
-    LibraryMissing,
+    GEN_UNIFORM, GEN_YEAR_OF_DAY, HT_PHT1, HT_PHT2, HT_TEXTBOOK, METHOD_LIP, METHOD_LIP_ADAPTIVE,
+    METHOD_NAIVE, MAX_DIMS, LibraryMissing,
 )
 
 METHODS = {"naive": METHOD_NAIVE, "lip": METHOD_LIP, "lip-adaptive": METHOD_LIP_ADAPTIVE}
@@ -65,13 +65,15 @@ def pred(*steps):
 
 
 def dim_desc(keys, n_rows, ht_len, attr=None, payload=None, predicate=(), keys_min=0,
-             bloom=None):
-    """Dimension descriptor.  bloom: None, ("mod", bits) or ("mult", bits, k, layout)."""
+             bloom=None, ht_kind=HT_PHT2):
+    """Dimension descriptor.  bloom: None, ("mod", bits) or ("mult", bits, k, layout);
+    ht_kind: HT_PHT2 (the reference's default), HT_PHT1 (keys only) or HT_TEXTBOOK
+    (open addressing with key compare; ht_len a power of two)."""
     d = _abi.DimDesc()
     d.keys, d.attr, d.payload = _ptr(keys), _ptr(attr), _ptr(payload)
     d.n_rows = int(n_rows)
     d.pred = predicate if isinstance(predicate, _abi.Pred) else pred(*predicate)
-    d.ht_kind = _abi.HT_PHT2
+    d.ht_kind = int(ht_kind)
     d.keys_min = int(keys_min)
     d.ht_len = int(ht_len)
     d.bloom_hash, d.bloom_layout, d.bloom_k, d.bloom_bits = BLOOM_NONE, 0, 0, 0

@@ -23,7 +23,7 @@ STATUS_NAMES = {
 COMB_SET, COMB_AND, COMB_OR = 0, 1, 2
 CMP_LT, CMP_GT, CMP_LTE, CMP_GTE, CMP_EQ = 0, 1, 2, 3, 4
 # tables / filters / methods
-HT_PHT2 = 0
+HT_PHT2, HT_PHT1, HT_TEXTBOOK = 0, 1, 2
 BLOOM_NONE, BLOOM_MOD, BLOOM_MULT = 0, 1, 2
 BLOOM_STANDARD, BLOOM_BLOCKED = 0, 1
 METHOD_NAIVE, METHOD_LIP, METHOD_LIP_ADAPTIVE = 0, 1, 2

@@ -17,9 +17,11 @@ Lineorder: custkey ~ U[1,C], suppkey ~ U[1,S], partkey ~ U[1,P],
 orderdate uniform over the 2556 datekeys, revenue ~ U[1,10^6],
 supplycost ~ U[1,10^5].
 
-Supplier's PHT_1 in Q2.x (q21/0-lip.cu:135, probed by BlockProbeAndPHT_1 at
-:77) is a PHT_2 here whose payload has group multiplier 0: with keys >= 1 both
-tables report a hit for exactly the inserted keys.  Q4.1 (q41/0-lip.cu) sums
+The dimensions the reference builds as PHT_1 (keys only, probed by
+BlockProbeAndPHT_1) are PHT_1 here too: the supplier in Q2.x (q21/0-lip.cu:77,
+135), supplier and part in Q4.1 (q41/0-lip.cu:88,100,157,186), the customer
+in Q4.2 (q42/0-naive.cu:62,128) and Q4.3 (q43/q43.cu:78,138).  None of them
+enters the group key; their payload is reported as 0.  Q4.1 (q41/0-lip.cu) sums
 (long long)(lo_revenue - lo_supplycost) (:127-128): the descriptor's
 fact_measure_sub column is gathered with the measure and subtracted per row.
 """
@@ -30,8 +32,8 @@ import numpy as np
 
 from . import _ptr
 from . import (CMP_EQ, CMP_GTE, CMP_LTE, COMB_AND, COMB_OR, COMB_SET, GEN_DATEKEY, GEN_DATEKEY_SEQ,
-               GEN_SEQUENCE, GEN_UNIFORM, GEN_YEAR_OF_DAY, dim_desc, gen_host, round_up_pow2,
-               star_desc)
+               GEN_SEQUENCE, GEN_UNIFORM, GEN_YEAR_OF_DAY, HT_PHT1, HT_PHT2, dim_desc, gen_host,
+               round_up_pow2, star_desc)
 
 DATE_MIN, DATE_MAX = 19920101, 19981230
 D_LEN = 2556
@@ -105,7 +107,7 @@ class DimQ:
     table: str          # "c", "s", "p" or "d"
     fk: str             # lineorder column joined to the table's key
     attr: str           # predicate column (None: every row selected)
-    payload: str        # PHT_2 value column
+    payload: str        # PHT_2 value column (None: a PHT_1 table, keys only)
     pred: tuple = ()    # ((comb, cmp, value), ...), crystal/pred.cuh semantics
 
 
@@ -129,7 +131,7 @@ _CITIES = ((COMB_SET, CMP_EQ, 231), (COMB_OR, CMP_EQ, 235))
 
 def _flight2(name, s_region, part_pred, source):
     # descriptor dims (S, P, D); g = (brand*7 + (year-1992)) % (7*1000)
-    return Query(name, (DimQ("s", "lo_suppkey", "s_region", "s_region", ((COMB_SET, CMP_EQ, s_region),)),
+    return Query(name, (DimQ("s", "lo_suppkey", "s_region", None, ((COMB_SET, CMP_EQ, s_region),)),
                         DimQ("p", "lo_partkey", part_pred[0], "p_brand1", part_pred[1]),
                         DimQ("d", "lo_orderdate", None, "d_year", ())),
                  (0, 7, 1), -1992, 7 * 5 * 5 * 40, 64, "SPD", source)
@@ -161,16 +163,16 @@ QUERIES = {
                     "q33/0-lip.cu:98-111,132-135,192-195"),
     # descriptor dims (S, C, P, D), the order of q41/0-{naive,lip}.cu;
     # g = (c_nation*7 + (year-1992)) % (7*25)
-    "q41": Query("q41", (DimQ("s", "lo_suppkey", "s_region", "s_region", ((COMB_SET, CMP_EQ, 1),)),
+    "q41": Query("q41", (DimQ("s", "lo_suppkey", "s_region", None, ((COMB_SET, CMP_EQ, 1),)),
                          DimQ("c", "lo_custkey", "c_region", "c_nation", ((COMB_SET, CMP_EQ, 1),)),
-                         DimQ("p", "lo_partkey", "p_mfgr", "p_mfgr",
+                         DimQ("p", "lo_partkey", "p_mfgr", None,
                               ((COMB_SET, CMP_EQ, 0), (COMB_OR, CMP_EQ, 1))),
                          DimQ("d", "lo_orderdate", None, "d_year", ())),
                  (0, 7, 0, 1), -1992, 7 * 25, 64, "SCPD",
                  "q41/0-lip.cu:120-128,151,157,178-186,209,217,237-243", measure_sub="lo_supplycost"),
     # q42/0-naive.cu: S->C->P->D; g = ((year-1992)*25*25 + s_nation*25 + category) % (7*25*25)
     "q42": Query("q42", (DimQ("s", "lo_suppkey", "s_region", "s_nation", ((COMB_SET, CMP_EQ, 1),)),
-                         DimQ("c", "lo_custkey", "c_region", "c_region", ((COMB_SET, CMP_EQ, 1),)),
+                         DimQ("c", "lo_custkey", "c_region", None, ((COMB_SET, CMP_EQ, 1),)),
                          DimQ("p", "lo_partkey", "p_mfgr", "p_category",
                               ((COMB_SET, CMP_EQ, 0), (COMB_OR, CMP_EQ, 1))),
                          DimQ("d", "lo_orderdate", "d_year", "d_year",
@@ -180,7 +182,7 @@ QUERIES = {
     # q43/q43.cu: S->C->P->D; g = ((year-1992)*250*1000 + s_city*1000 + brand) % (7*250*1000),
     # int32 atomicAdd of revenue - supplycost
     "q43": Query("q43", (DimQ("s", "lo_suppkey", "s_nation", "s_city", ((COMB_SET, CMP_EQ, 24),)),
-                         DimQ("c", "lo_custkey", "c_region", "c_region", ((COMB_SET, CMP_EQ, 1),)),
+                         DimQ("c", "lo_custkey", "c_region", None, ((COMB_SET, CMP_EQ, 1),)),
                          DimQ("p", "lo_partkey", "p_category", "p_brand1", ((COMB_SET, CMP_EQ, 3),)),
                          DimQ("d", "lo_orderdate", "d_year", "d_year",
                               ((COMB_SET, CMP_EQ, 1997), (COMB_OR, CMP_EQ, 1998)))),
@@ -226,8 +228,10 @@ def dim_descs(q, cols, sf, bloom=True):
         ht_len, kmin, bits = dim_params(dq, sf)
         n_rows = table_sizes(sf)[3] if dq.table == "d" else _ht_len(dq.table, sf)
         out.append(dim_desc(cols[f"{dq.table}_key"], n_rows, ht_len,
-                            attr=cols[dq.attr] if dq.attr else None, payload=cols[dq.payload],
-                            predicate=dq.pred, keys_min=kmin, bloom=("mod", bits) if bloom else None))
+                            attr=cols[dq.attr] if dq.attr else None,
+                            payload=cols[dq.payload] if dq.payload else None,
+                            predicate=dq.pred, keys_min=kmin, bloom=("mod", bits) if bloom else None,
+                            ht_kind=HT_PHT2 if dq.payload else HT_PHT1))
     return out
 
 
@@ -319,7 +323,7 @@ def main(argv=None):
     ctx = Context(args.device)
     if args.data_dir:
         need = {dq.fk for dq in q.dims} | {f"{dq.table}_key" for dq in q.dims} | {"lo_revenue"}
-        need |= {dq.attr for dq in q.dims if dq.attr} | {dq.payload for dq in q.dims}
+        need |= {dq.attr for dq in q.dims if dq.attr} | {dq.payload for dq in q.dims if dq.payload}
         if q.measure_sub:
             need.add(q.measure_sub)
         host = read_columns(args.data_dir, sorted(need))

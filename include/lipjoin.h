@@ -71,8 +71,18 @@ typedef struct {
 
 /* ---- dimension tables ----------------------------------------------------- */
 typedef enum {
-  LJ_HT_PHT2 = 0  /* crystal PHT_2: (key,val) int32 pairs, slot (key-min)%len,
-                     0 = empty, no key compare (join.cuh:108-189,260-335) */
+  LJ_HT_PHT2 = 0,     /* crystal PHT_2: (key,val) int32 pairs, slot (key-min)%len,
+                         0 = empty, no key compare (join.cuh:108-189,260-335) */
+  LJ_HT_PHT1 = 1,     /* crystal PHT_1: int32 keys only, slot (key-min)%len, hit iff the
+                         slot is nonzero, no payload (reported as 0) (join.cuh:29-106,
+                         191-258; the Q2.x / Q4.x suppliers, q21/0-lip.cu:77,135) */
+  LJ_HT_TEXTBOOK = 2  /* open addressing with key compare (not in the reference):
+                         ht_len slots (a power of two) of (key ^ 0x80000000, val) int32
+                         pairs, home slot (uint32)(key * 0x9E3779B1) >> (32 - log2 ht_len),
+                         linear probing; keys_min is not used by the table (the key
+                         INT32_MIN is outside its domain).  Agrees with PHT_2 on dense,
+                         nonzero, in-domain keys; differs on key 0 with value 0 and on
+                         fact keys outside the dimension's key range */
 } lj_ht_kind;
 
 typedef enum {
@@ -89,7 +99,7 @@ typedef enum {
 typedef struct {
   const int32_t* keys;     /* device, n_rows */
   const int32_t* attr;     /* device, predicate column (may be NULL if pred.n_steps == 0) */
-  const int32_t* payload;  /* device, PHT_2 value column; NULL -> attr */
+  const int32_t* payload;  /* device, PHT_2 / textbook value column; NULL -> attr */
   int64_t n_rows;
   lj_pred pred;
   int32_t ht_kind;         /* lj_ht_kind */
@@ -134,8 +144,10 @@ uint64_t lj_bytes_in_use(const lj_ctx* ctx);
  * The lj_dim stays valid until lj_dim_release. */
 lj_status lj_dim_build(lj_ctx* ctx, const lj_dim_desc* desc, lj_dim** out);
 lj_status lj_dim_get_info(lj_ctx* ctx, const lj_dim* dim, lj_dim_info* info);
-/* Copy the PHT (2*ht_len int32) and/or the Bloom words to host memory; either
- * pointer may be NULL.  Sizes are checked against the table sizes. */
+/* Copy the table (2*ht_len int32; PHT_1: ht_len) and/or the Bloom words to
+ * host memory; either pointer may be NULL.  Sizes are checked against the
+ * table sizes.  (A textbook table's layout depends on the order of the
+ * inserts under collisions; its contents as a map do not.) */
 lj_status lj_dim_export(lj_ctx* ctx, const lj_dim* dim, int32_t* ht_host, int64_t ht_words,
                         uint32_t* bloom_host, int64_t bloom_words);
 void lj_dim_release(lj_ctx* ctx, lj_dim* dim);

@@ -60,6 +60,24 @@ LJ_HD int64_t pht_slot(int32_t key, const FastMod& len, int32_t keys_min) {
   return (int64_t)fastmod_u32((uint32_t)x, len);
 }
 
+// Hash-table kinds of a dimension: the reference's perfect hashes (PHT_2
+// pairs, PHT_1 keys; crystal/join.cuh) and a textbook open-addressed table.
+enum HtKind : int32_t { HT_PHT2 = 0, HT_PHT1 = 1, HT_TEXTBOOK = 2 };
+
+// Textbook table: home slot of a key among mask + 1 (a power of two) slots,
+// Knuth's multiplicative hash (TAOCP vol. 3, sec. 6.4); keys are stored as
+// key ^ 0x80000000 so that an all-zero table is empty.
+LJ_HD uint32_t tb_home(int32_t key, uint32_t mask) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint32_t lg = (uint32_t)__popc(mask);
+#else
+  const uint32_t lg = (uint32_t)__builtin_popcount(mask);
+#endif
+  return lg == 0 ? 0u : ((uint32_t)key * 0x9E3779B1u) >> (32 - lg);
+}
+LJ_HD uint32_t tb_encode(int32_t key) { return (uint32_t)key ^ 0x80000000u; }
+constexpr int32_t kTbExcludedKey = (int32_t)0x80000000;  // INT32_MIN encodes to the empty slot
+
 // Multipliers of the multiplicative Bloom family: Knuth's golden-ratio
 // constant and the three odd 32-bit xxHash primes.
 #define LJ_MULT_A0 0x9E3779B1u

@@ -10,6 +10,7 @@
 #include "oracle.h"
 
 #include <pthread.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -107,6 +108,45 @@ int orc_pht2_probe(const int32_t* ht, int64_t len, int32_t keys_min, int32_t key
   return 1;
 }
 
+/* ---- textbook open addressing (project-defined, see oracle.h) ------------ */
+int64_t orc_tb_home(int32_t key, int64_t cap) {
+  int lg = 0;
+  while (((int64_t)1 << lg) < cap) lg++;
+  if (lg == 0) return 0;
+  return (int64_t)((uint32_t)((uint32_t)key * 0x9E3779B1u) >> (32 - lg));
+}
+
+void orc_tb_build(const int32_t* keys, const int32_t* vals, const uint8_t* flags, int64_t n,
+                  int32_t* ht, int64_t cap, int64_t* n_oob) {
+  int64_t oob = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (flags && !flags[i]) continue;
+    if (keys[i] == INT32_MIN) { oob++; continue; }
+    const int32_t enc = (int32_t)((uint32_t)keys[i] ^ 0x80000000u);
+    int64_t s = orc_tb_home(keys[i], cap), j = 0;
+    for (; j < cap; j++, s = (s + 1) & (cap - 1)) {
+      if (ht[2 * s] == 0) ht[2 * s] = enc;
+      if (ht[2 * s] == enc) { ht[2 * s + 1] = vals[i]; break; }
+    }
+    if (j == cap) oob++; /* table full */
+  }
+  if (n_oob) *n_oob += oob;
+}
+
+int orc_tb_probe(const int32_t* ht, int64_t cap, int32_t key, int32_t* payload) {
+  if (key == INT32_MIN) return 0;
+  const int32_t enc = (int32_t)((uint32_t)key ^ 0x80000000u);
+  int64_t s = orc_tb_home(key, cap);
+  for (int64_t j = 0; j < cap; j++, s = (s + 1) & (cap - 1)) {
+    if (ht[2 * s] == enc) {
+      if (payload) *payload = ht[2 * s + 1];
+      return 1;
+    }
+    if (ht[2 * s] == 0) return 0;
+  }
+  return 0;
+}
+
 /* ---- Bloom filters ------------------------------------------------------- */
 /* Odd 32-bit multipliers for the project-defined multiplicative family:
  * 0x9E3779B1 is Knuth's golden-ratio multiplier (TAOCP 6.4); the others are
@@ -185,7 +225,8 @@ typedef struct {
 } built_dim;
 
 /* build_kernel (join.cu:43-63, bloom_join.cu:43-65) and build_hashtable_{s,c,d}
- * (q33/0-lip.cu:114-203): predicate -> Bloom (if any) -> PHT_2(key -> payload). */
+ * (q33/0-lip.cu:114-203): predicate -> Bloom (if any) -> PHT_2(key -> payload);
+ * PHT_1 (q21/0-lip.cu:135) or the textbook table by the dimension's kind. */
 static int build_dim(const orc_dim* d, built_dim* out) {
   memset(out, 0, sizeof(*out));
   uint8_t* flags = (uint8_t*)malloc(d->n > 0 ? (size_t)d->n : 1);
@@ -196,10 +237,28 @@ static int build_dim(const orc_dim* d, built_dim* out) {
   orc_predicate(d->attr, d->n, d->pred, d->n_pred, flags);
   for (int64_t i = 0; i < d->n; i++) out->n_selected += flags[i];
   if (d->bloom.hash != ORC_BLOOM_NONE) orc_bloom_build(&d->bloom, d->keys, flags, d->n, out->bloom, &out->n_oob);
-  orc_pht2_build(d->keys, d->payload ? d->payload : d->attr, flags, d->n, out->ht, d->ht_len,
-                 d->keys_min, &out->n_oob);
+  const int32_t* vals = d->payload ? d->payload : d->attr;
+  if (d->ht_kind == ORC_HT_PHT1) orc_pht1_build(d->keys, flags, d->n, out->ht, d->ht_len, d->keys_min, &out->n_oob);
+  else if (d->ht_kind == ORC_HT_TEXTBOOK) orc_tb_build(d->keys, vals, flags, d->n, out->ht, d->ht_len, &out->n_oob);
+  else orc_pht2_build(d->keys, vals, flags, d->n, out->ht, d->ht_len, d->keys_min, &out->n_oob);
   free(flags);
   return 0;
+}
+
+/* The hash-table probe of a dimension of any kind (PHT_1 has no payload: 0). */
+static int probe_dim(const orc_dim* d, const built_dim* b, int32_t key, int32_t* pay) {
+  if (d->ht_kind == ORC_HT_PHT1) {
+    *pay = 0;
+    return orc_pht1_probe(b->ht, d->ht_len, d->keys_min, key);
+  }
+  if (d->ht_kind == ORC_HT_TEXTBOOK) return orc_tb_probe(b->ht, d->ht_len, key, pay);
+  return orc_pht2_probe(b->ht, d->ht_len, d->keys_min, key, pay);
+}
+
+/* A fact key outside the table's domain (the reference indexes out of bounds). */
+static int key_oob(const orc_dim* d, int32_t key) {
+  if (d->ht_kind == ORC_HT_TEXTBOOK) return key == INT32_MIN;
+  return orc_hash_pht(key, d->ht_len, d->keys_min) < 0;
 }
 
 static void free_dim(built_dim* b) {
@@ -232,9 +291,9 @@ static void* sj_worker(void* arg) {
     int32_t key = t->fk[i];
     if (use_bloom && !orc_bloom_probe(&d->bloom, t->bd->bloom, key)) continue;
     t->r.n_bloom_pass++;
-    if (orc_hash_pht(key, d->ht_len, d->keys_min) < 0) t->r.n_oob++;
+    if (key_oob(d, key)) t->r.n_oob++;
     int32_t pay = 0;
-    if (!orc_pht2_probe(t->bd->ht, d->ht_len, d->keys_min, key, &pay)) continue;
+    if (!probe_dim(d, t->bd, key, &pay)) continue;
     t->r.n_match++;
     int32_t m = t->measure[i];
     t->r.sum_measure += m;
@@ -334,19 +393,19 @@ static void* star_worker(void* arg) {
       t->rows[s]++;
       if (!line_alive[s]) { t->lines[s]++; line_alive[s] = 1; }
       key[d] = t->fact->fk[d][i];
-      if (orc_hash_pht(key[d], t->dims[d].ht_len, t->dims[d].keys_min) < 0) t->n_oob++;
+      if (key_oob(&t->dims[d], key[d])) t->n_oob++;
       if (t->method == ORC_METHOD_LIP) {
         if (t->dims[d].bloom.hash != ORC_BLOOM_NONE)
           alive = orc_bloom_probe(&t->dims[d].bloom, t->bd[d].bloom, key[d]);
       } else {
-        alive = orc_pht2_probe(t->bd[d].ht, t->dims[d].ht_len, t->dims[d].keys_min, key[d], &pay[d]);
+        alive = probe_dim(&t->dims[d], &t->bd[d], key[d], &pay[d]);
       }
     }
     if (alive && t->method == ORC_METHOD_LIP) {
       t->rows[nd + 1]++; /* passed every Bloom filter */
       for (int s = 0; s < nd && alive; s++) {
         int32_t d = t->order[s];
-        alive = orc_pht2_probe(t->bd[d].ht, t->dims[d].ht_len, t->dims[d].keys_min, key[d], &pay[d]);
+        alive = probe_dim(&t->dims[d], &t->bd[d], key[d], &pay[d]);
       }
     }
     if (!alive) continue;

@@ -20,6 +20,9 @@
  *                                      known values + no-false-negative / order-independence invariants
  *   orc_single_join                  : closed-form sums (all-selected, none-selected) + brute force
  *   orc_star_join                    : nested-loop SQL brute force (q33.sql semantics) on small inputs
+ *   orc_tb_build / orc_tb_probe      : project-defined (not in the reference): a dictionary of the
+ *                                      selected keys (last value wins), tables with collisions, a
+ *                                      full table, the excluded key
  *   line statistics                  : brute-force recount in tests
  */
 #ifndef LIPJOIN_ORACLE_H
@@ -67,6 +70,20 @@ void orc_pht2_build(const int32_t* keys, const int32_t* vals, const uint8_t* fla
 int orc_pht2_probe(const int32_t* ht, int64_t len, int32_t keys_min, int32_t key,
                    int32_t* payload);
 
+/* ---- textbook open addressing (not in the reference) ---------------------
+ * The hash table with key compare that the reference's perfect hash replaces
+ * (join.cuh:108-189 compares no keys).  cap slots (a power of two) of int32
+ * (stored key, value) pairs, stored key = key ^ 0x80000000 so that an
+ * all-zero table is empty (the key INT32_MIN is outside the domain); home
+ * slot = Knuth's multiplicative hash (TAOCP vol. 3 sec. 6.4)
+ * (uint32)(key * 0x9E3779B1) >> (32 - log2 cap); linear probing.  A
+ * repeated key keeps its slot and takes the last value. */
+int64_t orc_tb_home(int32_t key, int64_t cap);
+void orc_tb_build(const int32_t* keys, const int32_t* vals, const uint8_t* flags, int64_t n,
+                  int32_t* ht, int64_t cap, int64_t* n_oob);
+/* hit iff the key is stored; payload = its value */
+int orc_tb_probe(const int32_t* ht, int64_t cap, int32_t key, int32_t* payload);
+
 /* ---- Bloom filters: crystal/bloom.cuh ------------------------------------ */
 enum { ORC_BLOOM_NONE = 0, ORC_BLOOM_MOD = 1, ORC_BLOOM_MULT = 2 };
 enum { ORC_LAYOUT_STANDARD = 0, ORC_LAYOUT_BLOCKED = 1 };
@@ -93,6 +110,8 @@ int orc_bloom_bits(const orc_bloom* b, int32_t key, int64_t* bits_out);
 /* ---- whole-query drivers -------------------------------------------------- */
 enum { ORC_METHOD_NAIVE = 0, ORC_METHOD_LIP = 1 };
 
+enum { ORC_HT_PHT2 = 0, ORC_HT_PHT1 = 1, ORC_HT_TEXTBOOK = 2 };
+
 typedef struct {
   const int32_t* keys;
   const int32_t* attr;    /* predicate column (NULL only when n_pred == 0) */
@@ -101,8 +120,11 @@ typedef struct {
   const orc_pred_step* pred;
   int32_t n_pred;
   int32_t keys_min;
-  int64_t ht_len;
+  int64_t ht_len;         /* slots (TEXTBOOK: the capacity, a power of two) */
   orc_bloom bloom;        /* hash == NONE -> no Bloom filter */
+  int32_t ht_kind;        /* ORC_HT_*: PHT_2 (join.cuh:108-189,260-335), PHT_1 (no payload,
+                             join.cuh:29-106,191-258) or TEXTBOOK */
+  int32_t reserved0;
 } orc_dim;
 
 /* Single FK-PK join with an optional Bloom pre-probe

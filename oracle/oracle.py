@@ -30,10 +30,14 @@ class Bloom(C.Structure):
                 ("keys_min", C.c_int32), ("nbits", C.c_int64)]
 
 
+HT_PHT2, HT_PHT1, HT_TEXTBOOK = 0, 1, 2
+
+
 class Dim(C.Structure):
     _fields_ = [("keys", C.c_void_p), ("attr", C.c_void_p), ("payload", C.c_void_p),
                 ("n", C.c_int64), ("pred", C.POINTER(PredStep)), ("n_pred", C.c_int32),
-                ("keys_min", C.c_int32), ("ht_len", C.c_int64), ("bloom", Bloom)]
+                ("keys_min", C.c_int32), ("ht_len", C.c_int64), ("bloom", Bloom),
+                ("ht_kind", C.c_int32), ("reserved0", C.c_int32)]
 
 
 class JoinResult(C.Structure):
@@ -83,6 +87,11 @@ def load():
         lib.orc_single_join.restype = C.c_int
         lib.orc_single_join.argtypes = [C.POINTER(Dim), C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
                                         C.POINTER(JoinResult)]
+        lib.orc_tb_home.restype = C.c_int64
+        lib.orc_tb_home.argtypes = [C.c_int32, C.c_int64]
+        lib.orc_tb_build.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, i64p]
+        lib.orc_tb_probe.restype = C.c_int
+        lib.orc_tb_probe.argtypes = [C.c_void_p, C.c_int64, C.c_int32, i32p]
         lib.orc_star_join.restype = C.c_int64
         lib.orc_star_join.argtypes = [C.POINTER(Dim), C.c_int32, C.POINTER(Fact), i32p, i32p, C.c_int32, i64p,
                                       C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, i64p, i64p, i64p,
@@ -178,15 +187,36 @@ def pht1_probe(ht, ht_len, keys_min, key):
     return bool(load().orc_pht1_probe(ht.ctypes.data, int(ht_len), int(keys_min), int(key)))
 
 
+def tb_home(key, cap):
+    return load().orc_tb_home(int(key), int(cap))
+
+
+def tb_build(keys, vals, flags, cap):
+    keys, vals = _i32(keys), _i32(vals)
+    ht = np.zeros(2 * int(cap), dtype=np.int32)
+    oob = C.c_int64(0)
+    f = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+    load().orc_tb_build(keys.ctypes.data, vals.ctypes.data, None if f is None else f.ctypes.data, keys.size,
+                        ht.ctypes.data, int(cap), C.byref(oob))
+    return ht, oob.value
+
+
+def tb_probe(ht, cap, key):
+    pay = C.c_int32(0)
+    hit = load().orc_tb_probe(ht.ctypes.data, int(cap), int(key), C.byref(pay))
+    return (True, pay.value) if hit else (False, None)
+
+
 class DimSpec:
     """Host-side dimension: numpy columns + predicate + table/filter parameters."""
 
-    def __init__(self, keys, attr, ht_len, pred=(), payload=None, keys_min=0, bloom=None):
+    def __init__(self, keys, attr, ht_len, pred=(), payload=None, keys_min=0, bloom=None, ht_kind=HT_PHT2):
         self.keys = _i32(keys)
         self.attr = _i32(attr) if attr is not None else None
         self.payload = _i32(payload) if payload is not None else None
         self.ht_len, self.pred, self.keys_min = int(ht_len), list(pred), int(keys_min)
         self.bloom = bloom if bloom is not None else bloom_spec()
+        self.ht_kind = int(ht_kind)
         self._pred_arr = pred_array(self.pred)
 
     def c(self):
@@ -198,6 +228,7 @@ class DimSpec:
         d.pred = C.cast(self._pred_arr, C.POINTER(PredStep))
         d.n_pred = len(self.pred)
         d.keys_min, d.ht_len, d.bloom = self.keys_min, self.ht_len, self.bloom
+        d.ht_kind = self.ht_kind
         return d
 
 

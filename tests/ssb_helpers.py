@@ -14,8 +14,9 @@ def oracle_dims(q, host_dims, sf=SF):
     for dq in q.dims:
         ht_len, kmin, bits = ssb.dim_params(dq, sf)
         out.append(O.DimSpec(host_dims[f"{dq.table}_key"], host_dims[dq.attr] if dq.attr else None, ht_len,
-                             pred=list(dq.pred), payload=host_dims[dq.payload], keys_min=kmin,
-                             bloom=O.bloom_spec(O.BLOOM_MOD, bits, 1, keys_min=kmin)))
+                             pred=list(dq.pred), payload=host_dims[dq.payload] if dq.payload else None,
+                             keys_min=kmin, bloom=O.bloom_spec(O.BLOOM_MOD, bits, 1, keys_min=kmin),
+                             ht_kind=O.HT_PHT2 if dq.payload else O.HT_PHT1))
     return out
 
 
