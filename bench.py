@@ -75,6 +75,14 @@ def bench_batch(ctx, n_inst, reps):
         t.append(time.perf_counter() - t0)
     sep_s = statistics.median(t)
     out = {"instances": n_inst, "fact_rows_total": rows, "separate_ms": sep_s * 1e3}
+    # algorithmic bytes of one batch: per instance the FK column (4 B/row), the
+    # 128-byte measure lines holding a survivor, and the dimension build (key
+    # and attr read, 8 B/row; 8 B per inserted slot; the filter's words)
+    dim_bytes = sum(8 * sp["n_dim"] for sp in specs)
+    dim_bytes += sum(8 * round(sp["n_dim"] * sp["threshold"] / 10000.0) + ch["bloom_bits"] // 8
+                     for sp, ch in zip(specs, choices))
+    fact_bytes = sum(4 * sp["n_fact"] + 128 * r["stats"]["lines_in"][1] for sp, r in zip(specs, sep))
+    out["bytes_alg"] = fact_bytes + dim_bytes
     for mode, graph in (("eager", False), ("graph", True)):
         b = ctx.batch(descs, use_graph=graph)
         res = b.run()
@@ -90,6 +98,7 @@ def bench_batch(ctx, n_inst, reps):
         ms = statistics.median(t) * 1e3
         out[f"batch_{mode}_ms"] = ms
         out[f"batch_{mode}_rows_per_s"] = rows / (ms * 1e-3)
+        out[f"batch_{mode}_hbm_frac"] = out["bytes_alg"] / (ms * 1e-3) / HBM_PEAK
         out[f"speedup_{mode}_vs_separate"] = sep_s * 1e3 / ms
         out[f"bit_exact_{mode}"] = exact
     kinds = {}
@@ -151,12 +160,29 @@ def bench_single(ctx, reps=5):
         sp = {"n_dim": nd, "threshold": thr}
         bl, ch = W.tuned_bloom(ctx, sp)
         e["lip_tuned"] = dict(timed(dfk, dm, nf, L_dim(dk, nd, da, pred, bl), "lip"), bloom=str(bl))
+        # the tuned filter's words: popcount and a 64-bit hash (bit-exactness
+        # against the oracle is tested word for word)
+        dim = ctx.build_dim(L_dim(dk, nd, da, pred, bl))
+        _, words = dim.export()
+        e["lip_tuned"]["bloom_popcount"] = dim.info()["bloom_bits_set"]
+        e["lip_tuned"]["bloom_words_xxh64"] = words_hash(words)
+        dim.release()
         assert e["lip_tuned"]["sum"] == e["naive"]["sum"] == e["lip_best"]["sum"]
         rows.append(e)
         log(f"config2 s={sel}%: naive {e['naive']['probe_ms']:.3f} ms, LIP best {e['lip_best']['probe_ms']:.3f} "
             f"({best[0]}), tuned {e['lip_tuned']['probe_ms']:.3f} ({bl})")
     out["config2"] = rows
     return out
+
+
+def words_hash(words):
+    """XXH64 of a filter's words (little-endian bytes), hex."""
+    try:
+        import xxhash
+        return xxhash.xxh64(words.tobytes()).hexdigest()
+    except ImportError:
+        import hashlib
+        return "blake2b-" + hashlib.blake2b(words.tobytes(), digest_size=8).hexdigest()
 
 
 def L_dim(keys, n, attr, pred, bloom):

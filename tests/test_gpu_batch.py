@@ -152,3 +152,37 @@ def test_tune_bloom_loaded_filters(ctx):
     # nearly every row, so it must not be chosen (its modelled FPR is ~1)
     ch = ctx.tune_bloom(1 << 21, 1 << 22, 0.5)
     assert ch["bloom_bits"] >= 1 << 21 and ch["fpr"] < 0.5
+
+
+def test_batch_config4_full_shape_vs_oracle(ctx):
+    """The batched configuration at its own shape: 64 instances of 2^20..2^27
+    fact rows (log-uniform), each with its own 2^22-key dimension and
+    selectivity in [0.5, 20] %, the Bloom filter from the autotuner; one batch
+    (graph replay, run twice) checked instance by instance against the
+    oracle's single join."""
+    specs = W.batch_specs(SEED)
+    assert len(specs) == 64 and specs[0]["n_dim"] == 1 << 22
+    _flush(f"config 4 full shape: {sum(s['n_fact'] for s in specs)} fact rows")
+    cols, descs, blooms = [], [], []
+    for sp in specs:
+        c = W.batch_instance_device(ctx, sp)
+        bloom, _ = W.tuned_bloom(ctx, sp)
+        cols.append(c)
+        blooms.append(bloom)
+        descs.append(W.batch_desc(c, sp, method="lip", bloom=bloom))
+    torch.cuda.synchronize()
+    b = ctx.batch(descs, use_graph=True)
+    res = b.run()
+    res2 = b.run()
+    b.close()
+    _flush("config 4 full shape: oracle")
+    for i, (sp, bl) in enumerate(zip(specs, blooms)):
+        keys, attr, fk, meas = W.batch_instance_host(sp)
+        ref = O.single_join(O.DimSpec(keys, attr, sp["n_dim"], pred=list(W.single_join_pred(sp["threshold"])),
+                                      bloom=_oracle_bloom(bl)), fk, meas, method=O.LIP, threads=16)
+        exp = (ref["n_match"], ref["sum_measure"], ref["n_bloom_pass"])
+        for r in (res[i], res2[i]):
+            assert r["status"] == L._abi.OK
+            assert (r["n_rows"], r["sum_total"], r["stats"]["rows_bloom_pass"]) == exp, (i, sp)
+    del cols
+    torch.cuda.empty_cache()

@@ -65,21 +65,65 @@ def test_config1_plain_join_full(ctx):
     assert (r["n_match"], r["sum_measure"], r["sum_product"]) == (ref["n_match"], ref["sum_measure"], ref["sum_product"])
 
 
-@pytest.mark.parametrize("bloom", [("mult", 1 << 22, 2, L.BLOOM_STANDARD), ("mod", 1 << 22)])
-def test_config2_bloom_join_full(ctx, bloom):
-    """2^22-row dimension, 2^28 fact rows, 10 % selectivity, LIP with a Bloom filter."""
-    _flush(f"config 2 full {bloom}")
+@pytest.fixture(scope="module")
+def config2(ctx):
+    """The Bloom-join configuration: 2^22-row dimension (dense keys in shuffled
+    row order, attr ~ U[0,9999]), 2^28 fact rows; host and device columns."""
     seed, nd, nf = W.seed_of(2), 1 << 22, 1 << 28
-    keys, attr, fk, meas = W.single_join_host(seed, nd, nf, 9999)
-    dk, da, dfk, dm = W.single_join_device(ctx, seed, nd, nf, 9999)
-    pred = W.single_join_pred(1000)
-    desc = L.dim_desc(dk, nd, nd, attr=da, predicate=pred, bloom=bloom)
-    r = ctx.hash_join(dfk, dm, nf, desc, method="lip")
-    ob = O.bloom_spec(O.BLOOM_MOD, bloom[1], 1) if bloom[0] == "mod" else \
-        O.bloom_spec(O.BLOOM_MULT, bloom[1], bloom[2], layout=bloom[3])
-    ref = O.single_join(O.DimSpec(keys, attr, nd, pred=list(pred), bloom=ob), fk, meas, method=O.LIP,
-                        threads=THREADS)
-    assert (r["n_selected"], r["n_bloom_pass"], r["n_match"], r["sum_measure"]) == \
-        (ref["n_selected"], ref["n_bloom_pass"], ref["n_match"], ref["sum_measure"])
-    del dfk, dm
+    host = W.single_join_host(seed, nd, nf, 9999)
+    dev = W.single_join_device(ctx, seed, nd, nf, 9999)
+    torch.cuda.synchronize()
+    yield nd, nf, host, dev
+    del dev
     torch.cuda.empty_cache()
+
+
+def _obloom(bloom):
+    if bloom is None:
+        return O.bloom_spec()
+    if bloom[0] == "mod":
+        return O.bloom_spec(O.BLOOM_MOD, bloom[1], 1)
+    return O.bloom_spec(O.BLOOM_MULT, bloom[1], bloom[2], layout=bloom[3] if len(bloom) > 3 else 0)
+
+
+def test_config2_filter_words_every_size_and_k(ctx, config2):
+    """Bit-exact filter words of the 2^22-key dimension at 10 % selectivity for
+    every filter of the configuration's sweep: m = 2^16..2^26 bits x k = 1..3
+    (multiplicative, standard), the blocked layout and the reference's modulo
+    hash (bloom_join.cu:128-142 sizes the filter; bloom.cuh:10-88 builds it)."""
+    nd, nf, (keys, attr, _, _), (dk, da, _, _) = config2
+    pred = W.single_join_pred(1000)
+    flags = O.predicate(attr, list(pred))
+    cases = [("mult", 1 << lg, k, L.BLOOM_STANDARD) for lg in range(16, 27) for k in (1, 2, 3)]
+    cases += [("mult", 1 << lg, k, L.BLOOM_BLOCKED) for lg in (16, 20, 24, 26) for k in (2, 3)]
+    cases += [("mod", 1 << 22), ("mod", 3_000_000), ("mod", 1 << 26)]
+    _flush(f"config 2 filter words: {len(cases)} filters")
+    for bloom in cases:
+        dim = ctx.build_dim(L.dim_desc(dk, nd, nd, attr=da, predicate=pred, bloom=bloom))
+        _, words = dim.export()
+        info = dim.info()
+        dim.release()
+        ref = O.bloom_build(_obloom(bloom), keys, flags)
+        assert np.array_equal(words, ref), bloom
+        assert info["bloom_bits_set"] == int(np.unpackbits(ref.view(np.uint8)).sum()), bloom
+        assert info["n_selected"] == int(flags.sum())
+
+
+@pytest.mark.parametrize("sel", [1, 5, 10, 25, 50])
+def test_config2_every_selectivity(ctx, config2, sel):
+    """Survivor counts and int64 sums at 2^28 rows for every selectivity of
+    the configuration, LIP with the autotuner's filter and naive."""
+    nd, nf, (keys, attr, fk, meas), (dk, da, dfk, dm) = config2
+    thr = 100 * sel
+    pred = W.single_join_pred(thr)
+    bloom, _ = W.tuned_bloom(ctx, {"n_dim": nd, "threshold": thr})
+    _flush(f"config 2 full s={sel}% {bloom}")
+    r = ctx.hash_join(dfk, dm, nf, L.dim_desc(dk, nd, nd, attr=da, predicate=pred, bloom=bloom), method="lip")
+    ref = O.single_join(O.DimSpec(keys, attr, nd, pred=list(pred), bloom=_obloom(bloom)), fk, meas, method=O.LIP,
+                        threads=THREADS)
+    assert (r["n_selected"], r["n_bloom_pass"], r["n_match"], r["sum_measure"], r["sum_product"]) == \
+        (ref["n_selected"], ref["n_bloom_pass"], ref["n_match"], ref["sum_measure"], ref["sum_product"])
+    if sel in (1, 50):
+        rn = ctx.hash_join(dfk, dm, nf, L.dim_desc(dk, nd, nd, attr=da, predicate=pred), method="naive")
+        assert (rn["n_match"], rn["sum_measure"], rn["sum_product"]) == \
+            (ref["n_match"], ref["sum_measure"], ref["sum_product"])
