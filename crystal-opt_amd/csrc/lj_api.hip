@@ -47,6 +47,7 @@ struct lj_dim {
   FastMod ht_len{};
   int64_t n_selected = -1;
   int64_t n_oob = -1;
+  int64_t n_past = -1;     // resident: filter bits set past the staged prefix (-1: not known)
   float time_build_ms = 0.f;
   // tables taken from the zero pool (cleared behind their last use) go back
   // there; tables cleared inside the query's own window (lj_hash_join, the
@@ -91,10 +92,12 @@ struct lj_ctx {
   int stats_cap = 0;
   void* host_buf = nullptr;  // pinned: header + first kHostGroups groups
   void* host_buf_dev = nullptr;  // the same memory, as the device addresses it
-  // tuning (environment overrides for sweeps: LJ_PROBE_WG, LJ_PROBE_VEC, LJ_LDS_POLICY)
+  // tuning (environment overrides for sweeps: LJ_PROBE_WG, LJ_PROBE_VEC, LJ_LDS_POLICY, LJ_GROUP_SLOTS)
   int probe_wg = 0;
   int probe_vec = 0;
   int lds_policy = -1;
+  int group_slots = 0;           // LJ_GROUP_SLOTS: the LDS group table's size, 0 = from the group space
+  int filter_prefix = 1;         // LJ_FILTER_PREFIX=0: stage whole modulo filters
   double clock_khz = 100000.0;  // device wall clock (kernel phase timestamps)
   // Bloom autotuner cost model, per fact row, relative to the scan of an LDS filter
   // cost model of the Bloom autotuner, per fact row, in units of a scan with
@@ -422,6 +425,8 @@ extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* 
   c->probe_wg = env_int("LJ_PROBE_WG", 0);
   c->probe_vec = env_int("LJ_PROBE_VEC", 0);
   c->lds_policy = env_int("LJ_LDS_POLICY", -1);
+  c->group_slots = env_int("LJ_GROUP_SLOTS", 0);
+  c->filter_prefix = env_int("LJ_FILTER_PREFIX", 1);
   // lj_star_join_noalloc's scratch on this device: its address, once per
   // device (the device is current), and the one clear of its zero region
   if (device < kNaMaxDevices) {
@@ -610,8 +615,23 @@ static void add_fill(FillArgs& f, void* p, int64_t words) {
   f.n++;
 }
 
+// Words of a dimension's filter that the probe stages in LDS.  A modulo
+// filter sets bit (key - keys_min) mod m, so the keys of the table's domain
+// [keys_min, keys_min + ht_len) set bits below ht_len only: that prefix is
+// staged (the build counts the selected keys whose bit is past it; with any,
+// the probe reads the filter's global words for bits past the prefix).
+// Other hashes spread their bits: the whole filter.
+static int64_t filter_prefix_words(const lj_ctx* c, const lj_dim* D) {
+  if (!D->bloom_words) return 0;
+  if (!c->filter_prefix || (D->bh.kind != BLOOM_MOD && D->bh.kind != BLOOM_MOD_POW2) ||
+      D->desc.ht_kind == LJ_HT_TEXTBOOK)
+    return D->bloom_words;
+  const int64_t bits = std::min<int64_t>(D->desc.ht_len, D->bloom_words * 32);
+  return std::min<int64_t>(D->bloom_words, ((bits + 31) / 32 + 3) & ~3ll);
+}
+
 // Queue fill + build of up to 4 allocated dimensions.
-static void set_build_dim(const lj_dim* D, BuildDim& bd) {
+static void set_build_dim(const lj_ctx* c, const lj_dim* D, BuildDim& bd) {
   bd.keys = D->desc.keys;
   bd.attr = D->desc.attr;
   bd.payload = D->desc.payload;
@@ -629,12 +649,14 @@ static void set_build_dim(const lj_dim* D, BuildDim& bd) {
   bd.bloom = D->bloom;
   bd.bh = D->bh;
   bd.counters = D->counters;
+  const int64_t pw = filter_prefix_words(c, D);
+  bd.past_bit = pw < D->bloom_words ? pw * 32 : 0;
 }
 
 // Build of allocated (all-zero) dimensions: one launch for all of them.
 static lj_status dims_launch(lj_ctx* c, lj_dim* const* dims, int n, long long* clock) {
   BuildArgs b{};
-  for (int i = 0; i < n; i++) set_build_dim(dims[i], b.dim[b.n_dims++]);
+  for (int i = 0; i < n; i++) set_build_dim(c, dims[i], b.dim[b.n_dims++]);
   b.clock = clock;
   HIP_TRY(c, launch_build(b, c->stream));
   return LJ_OK;
@@ -653,10 +675,11 @@ extern "C" lj_status lj_dim_build(lj_ctx* c, const lj_dim_desc* desc, lj_dim** o
     return s;
   }
   HIP_TRY(c, hipEventRecord(c->ev[1], c->stream));
-  unsigned long long hc[2 * kCounterLanes], h[2];
+  unsigned long long hc[kCounterWords], h[2];
   HIP_TRY(c, hipMemcpyAsync(hc, D->counters, sizeof hc, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   sum_counters(hc, h);
+  D->n_past = (int64_t)hc[kCounterPast];
   HIP_TRY(c, hipEventElapsedTime(&D->time_build_ms, c->ev[0], c->ev[1]));
   D->n_selected = (int64_t)h[0];
   D->n_oob = (int64_t)h[1];
@@ -869,7 +892,8 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   ProbeArgs& a = p.a;
   a = ProbeArgs{};
   int lds_words = 0;
-  const int64_t lds_budget_words = ((int64_t)c->max_lds - (int64_t)probe_lds_bytes(cfg, 0) - 16) / 4;
+  const int64_t lds_budget_words =
+      ((int64_t)c->max_lds - (int64_t)probe_lds_bytes(cfg, 0, kGroupSlots) - 16) / 4;
   for (int d = 0; d < nd; d++) {
     ProbeDim& P = a.dim[d];
     lj_dim* D = p.dims[d];
@@ -885,19 +909,72 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
     P.keys_min = D->desc.ht_kind == LJ_HT_TEXTBOOK ? kTbExcludedKey + 1 : D->desc.keys_min;
     P.bloom_lds = -1;
     P.bloom_words = (int32_t)D->bloom_words;
+    P.past_count = nullptr;
+    P.bloom_lds_words = (int32_t)D->bloom_words;
     P.gmult = q->group_mult[d];
   }
+  // Whole filters, or their prefixes (filter_prefix_words): prefixes free
+  // LDS for a larger group table when the group space is small enough for
+  // every group to be hit (at most kRepGroups), and let more filters be
+  // staged when the whole ones do not all fit.  Otherwise whole filters: no
+  // prefix check at the probe's start, the small group table (a large one
+  // costs a 437,500-group space with 24 groups hit more than it saves).
+  const bool small_space = q->n_groups > 1 && q->n_groups <= kRepGroups;
+  bool use_prefix = small_space;
+  if (lip && !use_prefix) {
+    int64_t need = 0;
+    for (int d = 0; d < nd; d++)
+      if (a.dim[d].bh.kind != BLOOM_NONE && a.dim[d].bloom) need += (a.dim[d].bloom_words + 3) & ~3ll;
+    use_prefix = need > lds_budget_words;
+  }
+  if (lip && use_prefix) {
+    // a resident dimension with bits past its prefix stages the whole
+    // filter; one built with the query passes its counter to the probe
+    // (which checks it once, at its start)
+    for (int d = 0; d < nd; d++) {
+      ProbeDim& P = a.dim[d];
+      lj_dim* D = p.dims[d];
+      int64_t pw = filter_prefix_words(c, D);
+      if (pw < D->bloom_words) {
+        if (q->resident[d] && D->n_past != 0) pw = D->bloom_words;
+        else if (!q->resident[d]) P.past_count = D->counters + kCounterPast;
+      }
+      P.bloom_lds_words = (int32_t)pw;
+    }
+  }
   if (lip) {
-    // place filters in LDS in load order; policy 1 keeps only the first one
+    // Filters staged in LDS (their staged prefix), in load order.  Policy 0:
+    // every filter that fits.  Policy 1: the stage-0 filter only.  Policy 2:
+    // a static plan's stage-0 filter and every filter of at most
+    // kHotFilterWords, larger ones later in the chain probed from L2 (the
+    // adaptive order stages the small filters, or the largest that fits if
+    // none is small).  Measured on Q3.3 and SSB flights 2-4: policy 0 is the
+    // fastest (a 128 KiB filter probed from L2 by a fifth of the rows costs
+    // more than the group table its LDS would hold saves).
     for (int st = 0; st < nd; st++) {
       const int d = adaptive ? st : p.bloom_order[st];
       ProbeDim& P = a.dim[d];
       if (P.bh.kind == BLOOM_NONE || !P.bloom) continue;
-      const int64_t w4 = (P.bloom_words + 3) & ~3ll;
+      const int64_t w4 = (P.bloom_lds_words + 3) & ~3ll;
       if (lds_words + w4 > lds_budget_words) continue;
       if (lds_policy == 1 && !adaptive && st > 0) continue;
+      if (lds_policy == 2 && w4 > kHotFilterWords && (adaptive || st > 0)) continue;
       P.bloom_lds = lds_words;
       lds_words += (int)w4;
+    }
+    if (adaptive && lds_words == 0) {
+      int best = -1;
+      for (int d = 0; d < nd; d++) {
+        const ProbeDim& P = a.dim[d];
+        const int64_t w4 = (P.bloom_lds_words + 3) & ~3ll;
+        if (P.bh.kind != BLOOM_NONE && P.bloom && w4 <= lds_budget_words &&
+            (best < 0 || w4 > ((a.dim[best].bloom_lds_words + 3) & ~3ll)))
+          best = d;
+      }
+      if (best >= 0) {
+        a.dim[best].bloom_lds = 0;
+        lds_words = (int)((a.dim[best].bloom_lds_words + 3) & ~3ll);
+      }
     }
   }
   // stage-0 kernel specialisation: the kind of the first filter (every
@@ -938,12 +1015,28 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
       }
     }
   }
-  // every filter of the kind (and placement) of stage 0: the kernel probes
-  // later stages with that kind's short test
-  a.all_k0 = lip && cfg.k0 != S0_GENERIC ? 1 : 0;
+  // every filter of stage 0's hash family (in LDS or not): the kernel
+  // probes later stages with that family's short test
+  auto family = [](int k) { return k == S0_MODP2_LDS || k == S0_MODP2_GL ? 1 : k == S0_MOD_LDS || k == S0_MOD_GL ? 2
+                                   : k == S0_MULT_LDS || k == S0_MULT_GL ? 3 : 0; };
+  a.all_k0 = lip && family(cfg.k0) != 0 ? 1 : 0;
   for (int d = 0; d < nd; d++)
-    if (lip && kind_of(d) != cfg.k0) a.all_k0 = 0;
-  p.lds_bytes = probe_lds_bytes(cfg, lds_words);
+    if (lip && family(kind_of(d)) != family(cfg.k0)) a.all_k0 = 0;
+  // the LDS group table: 32 slots, or for a small group space up to twice
+  // its size (at most kGroupSlotsMax) in the LDS that staging leaves.  A
+  // 1024-thread workgroup is alone on its CU (the probe's registers allow 16
+  // waves per CU), so the rest of the CU's LDS is free; smaller workgroups
+  // keep the small table.
+  int gslots = kGroupSlots;
+  const int env_slots = c->group_slots;
+  if ((small_space && cfg.wg == 1024) || (q->n_groups > 1 && env_slots > 0)) {
+    int64_t want = std::min<int64_t>(kGroupSlotsMax, 2 * q->n_groups);
+    if (env_slots > 0) want = env_slots;
+    const int64_t left = (int64_t)c->max_lds - (int64_t)probe_lds_bytes(cfg, lds_words, 0) - 16;
+    while ((int64_t)gslots * 2 <= want && (int64_t)gslots * 2 * kGroupSlotBytes <= left) gslots *= 2;
+  }
+  a.gslot_bits = ilog2_u64((uint64_t)gslots);
+  p.lds_bytes = probe_lds_bytes(cfg, lds_words, gslots);
   lj_status s = occupancy_of(c, cfg, p.lds_bytes, &p.per_cu, cache_occupancy);
   if (s != LJ_OK) return s;
   a.measure = q->fact_measure;
@@ -1521,7 +1614,7 @@ static lj_status batch_build(lj_ctx* c, const lj_star_desc* descs, int32_t n, co
   for (BatchQuery& bq : b->q)
     for (int i = 0; i < bq.p.n_owned; i++) {
       BuildDim bd{};
-      set_build_dim(bq.p.owned[i], bd);
+      set_build_dim(c, bq.p.owned[i], bd);
       bd.block_begin = b->build_blocks;
       b->build_blocks += build_blocks(bd.n);
       bds.push_back(bd);
@@ -1729,7 +1822,7 @@ extern "C" lj_status lj_tune_bloom(lj_ctx* c, int64_t n_selected, int64_t n_dim_
     return fail(c, LJ_EINVAL, "n_selected %lld, n_dim_rows %lld, fact_pass %g", (long long)n_selected,
                 (long long)n_dim_rows, fact_pass);
   ProbeConfig cfg{1024, 1, S0_MULT_LDS};
-  const int64_t lds_bits = ((int64_t)c->max_lds - (int64_t)probe_lds_bytes(cfg, 0) - 16) * 8;
+  const int64_t lds_bits = ((int64_t)c->max_lds - (int64_t)probe_lds_bytes(cfg, 0, kGroupSlots) - 16) * 8;
   lj_bloom_choice best{};
   best.cost = 1e300;
   struct Cand { int hash, layout, k; };

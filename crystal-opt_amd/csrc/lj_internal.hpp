@@ -11,7 +11,11 @@
 namespace lj {
 
 constexpr int kMaxDims = 4;
-constexpr int kGroupSlots = 32;      // LDS-privatised groups per workgroup
+constexpr int kMaxDimsConst = kMaxDims;
+constexpr int kGroupSlots = 32;      // LDS-privatised groups per workgroup (the smallest table)
+constexpr int kGroupSlotsMax = 4096; // the largest table (sized from the LDS that staging leaves)
+constexpr int kGroupSlotBytes = 3 * 8 + 4 + kMaxDimsConst * 4;  // sum, count, product, key, payloads
+constexpr int kHotFilterWords = 4096; // filters up to 16 KB are staged in LDS wherever they sit in the chain
 // Group spaces up to kRepGroups keep kGroupReps copies of the global group
 // accumulators (copy = workgroup index mod 8, i.e. one per XCD under the
 // round-robin dispatch), so that groups overflowing the LDS table are not all
@@ -53,15 +57,17 @@ struct BuildDim {
   int32_t ht_kind;         // HtKind
   uint32_t* bloom;         // nullptr: no filter
   BloomHash bh;
-  unsigned long long* counters;  // kCounterLanes x {selected rows, keys outside the domain}
+  unsigned long long* counters;  // kCounterLanes x {selected rows, keys outside the domain}, ..., kCounterPast
   int64_t block_begin;     // first workgroup of this dimension in the build launch
+  int64_t past_bit;        // modulo filter: the first bit past its staged prefix (0: not counted)
 };
 
 // A dimension's counters are spread over kCounterLanes pairs (workgroup w adds
 // to pair w % kCounterLanes) so that the build's per-workgroup atomics do not
 // all serialize on one address; readers sum the pairs.
 constexpr int kCounterLanes = 16;
-constexpr int kCounterWords = 2 * kCounterLanes + 2;  // + popcount scratch
+constexpr int kCounterWords = 2 * kCounterLanes + 2;  // + popcount scratch, + kCounterPast
+constexpr int kCounterPast = 2 * kCounterLanes + 1;   // selected keys whose filter bit is past the staged prefix
 
 struct ProbeDim {
   const int32_t* fk;      // fact FK column of this dimension
@@ -73,7 +79,11 @@ struct ProbeDim {
   int32_t bloom_lds;      // word offset of the staged copy in LDS, -1 = global
   int32_t bloom_words;
   int32_t ht_kind;        // HtKind
+  int32_t bloom_lds_words;// words staged: a prefix of the filter; a word past it is read from
+                          // global memory (only out-of-domain keys reach it: see plan_probe)
   int64_t gmult;
+  const unsigned long long* past_count;  // filter bits set past the staged prefix (device
+                                         // counter of a dimension built with the query), or null
 };
 
 struct ProbeArgs {
@@ -88,7 +98,8 @@ struct ProbeArgs {
   int32_t order[kMaxDims];        // hash-probe (join) order
   int32_t load_order[kMaxDims];   // column load order (NAIVE: order, LIP: Bloom order)
   int32_t lds_filter_words;
-  int32_t all_k0;          // LIP: every dimension's filter has stage 0's kind and placement
+  int32_t all_k0;          // LIP: every dimension's filter has stage 0's hash family (the
+                           // placement, LDS or global, may differ: a uniform branch)
   uint32_t s0_mask;        // adaptive: dimensions the sample may put first (their filters have
                            // stage 0's kind); 0 = any
   int64_t goffset;
@@ -100,7 +111,7 @@ struct ProbeArgs {
   uint32_t* touched;       // first-touched groups, in touch order
   uint32_t* n_touched;
   uint32_t touched_cap;
-  int32_t pad1;
+  int32_t gslot_bits;      // log2 of the workgroup's LDS group-table slots
   long long* stats;        // [gridDim.x * kStats]
   // result: the last workgroup to finish copies the groups out and resets the
   // dense accumulators
@@ -194,7 +205,7 @@ int probe_entries_mod(const ProbeEntry** out);
 int probe_entries_mult(const ProbeEntry** out);
 int probe_entries_generic(const ProbeEntry** out);
 
-size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words);
+size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words, int group_slots);
 hipError_t probe_occupancy(const ProbeConfig& c, size_t lds_bytes, int* blocks_per_cu);
 // A batch: qs[n_q] and wg_begin[n_q + 1] in device memory; grid == wg_begin[n_q].
 hipError_t launch_probe_multi(const ProbeConfig& c, const ProbeArgs* qs, const uint32_t* wg_begin, int n_q, int grid,

@@ -85,12 +85,12 @@ __device__ __forceinline__ bool tb_insert(int32_t* ht, uint32_t mask, int32_t ke
 
 // Workgroup `blk` of dimension B: rows [blk * 256 * rpt, (blk + 1) * 256 * rpt).
 __device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
-  __shared__ unsigned int cnt[2];
-  if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+  __shared__ unsigned int cnt[3];
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
   __syncthreads();
   const int rpt = build_rows_per_thread(B.n);
   const int64_t r0 = (blk * kBuildThreads + threadIdx.x) * rpt;
-  uint32_t nsel = 0, noob = 0;
+  uint32_t nsel = 0, noob = 0, npast = 0;
 #pragma unroll
   for (int j = 0; j < kBuildRows; j++) {
     const int64_t r = r0 + j;
@@ -99,7 +99,13 @@ __device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
       const bool sel = B.pred.n_steps == 0 || eval_pred(B.pred, B.attr[r]);
       if (sel) {
         nsel++;
-        if (B.bloom) bloom_insert(B.bh, B.bloom, key);
+        if (B.bloom) {
+          bloom_insert(B.bh, B.bloom, key);
+          if (B.past_bit) {
+            uint32_t bits[3];
+            if (bloom_bits(B.bh, key, bits) && (int64_t)bits[0] >= B.past_bit) npast++;
+          }
+        }
         if (B.ht_kind == HT_TEXTBOOK) {
           if (!tb_insert(B.ht, B.ht_len.pow2_mask, key, B.payload[r])) noob++;
         } else {
@@ -118,15 +124,18 @@ __device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
   }
   nsel = WaveSum(nsel);
   noob = WaveSum(noob);
+  npast = WaveSum(npast);
   if (__lane_id() == 0) {
     if (nsel) atomicAdd(&cnt[0], nsel);
     if (noob) atomicAdd(&cnt[1], noob);
+    if (npast) atomicAdd(&cnt[2], npast);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long* c = B.counters + 2 * (blockIdx.x % kCounterLanes);
     if (cnt[0]) atomicAdd(&c[0], (unsigned long long)cnt[0]);
     if (cnt[1]) atomicAdd(&c[1], (unsigned long long)cnt[1]);
+    if (cnt[2]) atomicAdd(&B.counters[kCounterPast], (unsigned long long)cnt[2]);
   }
 }
 

@@ -6,10 +6,10 @@
 
 namespace lj {
 
-size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words) {
+size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words, int group_slots) {
   size_t off = ((size_t)lds_filter_words * 4 + 15) & ~(size_t)15;
   off += (size_t)(c.wg / 64) * 3 * kQueueCap * 4;
-  off += 3 * kGroupSlots * 8 + kGroupSlots * 4 + kGroupSlots * kMaxDims * 4;
+  off += (size_t)group_slots * kGroupSlotBytes;
   off += kStats * 8 + (kMaxDims + 1) * 8;
   off += 16;  // tile counter
   return off;

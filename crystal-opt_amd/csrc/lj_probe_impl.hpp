@@ -46,8 +46,8 @@
 namespace lj {
 
 constexpr int kDenseMin = 8;         // stage-0 survivors of a segment above which it is processed in place
-constexpr int kGroupSlotBits = 5;
-static_assert((1 << kGroupSlotBits) == kGroupSlots, "group slots: a power of two");
+constexpr int kGroupProbes = 32;     // slots a row tries in the LDS group table before going global
+static_assert(kGroupSlotBytes == 3 * 8 + 4 + kMaxDims * 4, "group slot layout");
 static_assert(kQueue + kDenseMin <= kQueueCap, "below kQueue entries, a sparse segment always fits");
 static_assert((kQueueCap - kQueue) + 16 * 4 <= kQueueCap, "after a rotation, a dense part always fits");
 constexpr int32_t kEmptyGroup = -1;
@@ -99,9 +99,53 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T* gl(const T
 // (LDS copy or global words, a uniform choice): a pointer that could be
 // either would compile to flat loads, each followed by a wait for all of the
 // wave's outstanding memory operations (the stream's prefetched tiles too).
+// Bits of a filter staged as a prefix: the LDS copy below bloom_lds_words,
+// the global words past it.  FB false (the probe checked at its start that no
+// bit is set past any staged prefix): a bit past the prefix is 0, and only
+// LDS is read.  FB true: a bit past the prefix is read from the global words
+// (a global read in this code makes the wave wait for its vector-memory
+// counter, the stream's prefetched tiles included, so the fast body has none).
+template <bool FB>
+__device__ __forceinline__ bool staged_bit(const ProbeDim& D, const uint32_t* lds, uint32_t h) {
+  const uint32_t P = (uint32_t)D.bloom_lds_words, w = h >> 5;
+  const LdsWords f = (LdsWords)(lds + D.bloom_lds);
+  if constexpr (FB) {
+    return ((w < P ? f[w] : ((GlobalWords)D.bloom)[w]) >> (h & 31)) & 1u;
+  } else {
+    return w < P && ((f[w < P ? w : 0u] >> (h & 31)) & 1u);
+  }
+}
+
+// The 4 bits h[j] of a lane's rows, as a mask.
+template <bool FB>
+__device__ __forceinline__ uint32_t staged_bits4(const ProbeDim& D, const uint32_t* lds, const uint32_t (&h)[4]) {
+  const uint32_t P = (uint32_t)D.bloom_lds_words;
+  const LdsWords f = (LdsWords)(lds + D.bloom_lds);
+  uint32_t w[4], m = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const uint32_t wi = h[j] >> 5;
+    if constexpr (FB) w[j] = wi < P ? f[wi] : ((GlobalWords)D.bloom)[wi];
+    else w[j] = f[wi < P ? wi : 0u];
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    uint32_t b = __builtin_amdgcn_ubfe(w[j], h[j], 1u);
+    if constexpr (!FB) b &= (uint32_t)((h[j] >> 5) < P);
+    m |= b << j;
+  }
+  return m;
+}
+
 __device__ __forceinline__ bool bloom_probe_rt(const ProbeDim& D, const uint32_t* lds, int32_t key) {
   if (D.bh.kind == BLOOM_NONE) return true;
-  if (D.bloom_lds >= 0) return bloom_test(D.bh, (LdsWords)(lds + D.bloom_lds), key);
+  if (D.bloom_lds >= 0) {
+    if (D.bloom_lds_words == D.bloom_words) return bloom_test(D.bh, (LdsWords)(lds + D.bloom_lds), key);
+    // a staged prefix (a modulo hash: one bit per key)
+    uint32_t bits[3];
+    if (bloom_bits(D.bh, key, bits) == 0) return false;
+    return staged_bit<true>(D, lds, bits[0]);
+  }
   return bloom_test(D.bh, (GlobalWords)D.bloom, key);
 }
 
@@ -164,23 +208,67 @@ __device__ __forceinline__ bool probe_filter(const ProbeArgs& a, const ProbeDim&
       uint32_t h;
       if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) h = x & D.bh.mod.pow2_mask;
       else h = fastmod_u32(x, D.bh.mod);
-      uint32_t word;
-      if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS) word = ((LdsWords)(lds + D.bloom_lds))[h >> 5];
-      else word = ((GlobalWords)D.bloom)[h >> 5];
-      return key >= D.bh.keys_min && ((word >> (h & 31)) & 1u);
+      bool bit;
+      if (D.bloom_lds >= 0) bit = staged_bit<SPEC != 1>(D, lds, h);
+      else bit = (((GlobalWords)D.bloom)[h >> 5] >> (h & 31)) & 1u;
+      return key >= D.bh.keys_min && bit;
     }
-  } else if constexpr (K0 == S0_MULT_LDS) {
-    if (spec) return bloom_test(D.bh, (LdsWords)(lds + D.bloom_lds), key);
-  } else if constexpr (K0 == S0_MULT_GL) {
-    if (spec) return bloom_test(D.bh, (GlobalWords)D.bloom, key);
+  } else if constexpr (K0 == S0_MULT_LDS || K0 == S0_MULT_GL) {
+    if (spec) {
+      if (D.bloom_lds >= 0) return bloom_test(D.bh, (LdsWords)(lds + D.bloom_lds), key);
+      return bloom_test(D.bh, (GlobalWords)D.bloom, key);
+    }
   }
   return bloom_probe_rt(D, lds, key);
+}
+
+// The filter probes of a lane's 4 rows (bit j of mask4: row j alive): with
+// the plan's hash family, the 4 words are read together (one wait) and a
+// dead row's read is harmless (its bit index is in range whatever its key).
+// Returns the surviving rows.
+template <int K0, int SPEC>
+__device__ __forceinline__ uint32_t probe_filter4(const ProbeArgs& a, const ProbeDim& D, const uint32_t* lds,
+                                                  const v4i32& kv, uint32_t mask4) {
+  const bool spec = SPEC == 1 || (SPEC == 0 && a.all_k0);
+  if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_GL) {
+    if (spec) {
+      const int32_t kk[4] = {kv.x, kv.y, kv.z, kv.w};
+      uint32_t h[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t x = (uint32_t)kk[j] - (uint32_t)D.bh.keys_min;
+        if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) h[j] = x & D.bh.mod.pow2_mask;
+        else h[j] = fastmod_u32(x, D.bh.mod);
+      }
+      uint32_t ok = 0;
+      if (D.bloom_lds >= 0) {
+        ok = staged_bits4<SPEC != 1>(D, lds, h);
+      } else {
+        const GlobalWords f = (GlobalWords)D.bloom;
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) w[j] = f[h[j] >> 5];
+#pragma unroll
+        for (int j = 0; j < 4; j++) ok |= __builtin_amdgcn_ubfe(w[j], h[j], 1u) << j;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) ok &= ~((uint32_t)(kk[j] < D.bh.keys_min) << j);
+      return ok & mask4;
+    }
+  }
+  uint32_t ok = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if ((mask4 >> j) & 1u) ok |= (uint32_t)probe_filter<K0, SPEC>(a, D, lds, vget(kv, j)) << j;
+  return ok;
 }
 
 // Stage-0 parameters, hoisted out of the streaming loop.
 struct S0 {
   const int32_t* fk;
   const uint32_t* bf;    // staged copy in LDS or the global words
+  const uint32_t* gbf;   // the global words
+  uint32_t lds_words;    // words of bf that are staged (a prefix); the rest is read from gbf
   const int32_t* ht;
   BloomHash bh;
   FastMod fm;
@@ -196,7 +284,7 @@ __device__ __forceinline__ bool s0_probe(const S0& s, int32_t key, int32_t& pay)
 // unconditionally (a masked bit index is always inside the filter; an invalid
 // PHT slot reads slot 0), so the 4 lookups are in flight together and the
 // result is fixed up with selects.  Returns the alive mask (bit j = row j).
-template <int K0>
+template <int K0, bool FB>
 __device__ __forceinline__ uint32_t s0_probe4(const S0& s, const v4i32& k, uint32_t valid4, int32_t (&pay)[4]) {
   const int32_t kk[4] = {k.x, k.y, k.z, k.w};
   uint32_t ok = 0;
@@ -214,19 +302,36 @@ __device__ __forceinline__ uint32_t s0_probe4(const S0& s, const v4i32& k, uint3
       if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) h[j] = x;
       else h[j] = fastmod_u32(x, s.bh.mod);
     }
+    uint32_t wi[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL)
-        wv[j] = s.bf[__builtin_amdgcn_ubfe(h[j], 5u, wbits)];
-      else
-        wv[j] = s.bf[h[j] >> 5];
+      if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) wi[j] = __builtin_amdgcn_ubfe(h[j], 5u, wbits);
+      else wi[j] = h[j] >> 5;
+    }
+    // v_bfe_u32 reads bits [4:0] of the offset: bit h & 31 of the word
+    if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS) {
+      // the staged prefix (see staged_bit): FB reads a word past it from the
+      // global words; otherwise such a bit is 0
+      const uint32_t P = s.lds_words;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if constexpr (FB) wv[j] = wi[j] < P ? ((LdsWords)s.bf)[wi[j]] : ((GlobalWords)s.gbf)[wi[j]];
+        else wv[j] = ((LdsWords)s.bf)[wi[j] < P ? wi[j] : 0u];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        uint32_t b = __builtin_amdgcn_ubfe(wv[j], h[j], 1u);
+        if constexpr (!FB) b &= (uint32_t)(wi[j] < P);
+        ok |= b << j;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++) wv[j] = ((GlobalWords)s.bf)[wi[j]];
+#pragma unroll
+      for (int j = 0; j < 4; j++) ok |= __builtin_amdgcn_ubfe(wv[j], h[j], 1u) << j;
     }
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      // v_bfe_u32 reads bits [4:0] of the offset: bit h & 31 of the word
-      ok |= __builtin_amdgcn_ubfe(wv[j], h[j], 1u) << j;
-      pay[j] = 0;
-    }
+    for (int j = 0; j < 4; j++) pay[j] = 0;
   } else if constexpr (K0 == S0_PHT) {
     if (s.ht_kind == HT_PHT2) {
       uint64_t slot[4];
@@ -277,18 +382,24 @@ __device__ __forceinline__ bool s0_probe(const S0& s, int32_t key, int32_t& pay)
     return sl >= 0 && ht_resolve(s.ht_kind, s.ht, s.fm, key, sl, ht_load(s.ht_kind, s.ht, sl), pay);
   } else if constexpr (K0 == S0_NONE) {
     return true;
-  } else if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) {
+  } else if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_LDS || K0 == S0_MOD_GL) {
     const uint32_t x = (uint32_t)key - (uint32_t)s.bh.keys_min;
-    const uint32_t h = x & s.mask;
-    return key >= s.bh.keys_min && ((s.bf[h >> 5] >> (h & 31)) & 1u);
-  } else if constexpr (K0 == S0_MOD_LDS || K0 == S0_MOD_GL) {
-    const uint32_t x = (uint32_t)key - (uint32_t)s.bh.keys_min;
-    const uint32_t h = fastmod_u32(x, s.bh.mod);
-    return key >= s.bh.keys_min && ((s.bf[h >> 5] >> (h & 31)) & 1u);
+    uint32_t h;
+    if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MODP2_GL) h = x & s.mask;
+    else h = fastmod_u32(x, s.bh.mod);
+    const uint32_t w = h >> 5;
+    const uint32_t word = w < s.lds_words ? s.bf[w] : s.gbf[w];
+    return key >= s.bh.keys_min && ((word >> (h & 31)) & 1u);
   } else if constexpr (K0 == S0_MULT_LDS || K0 == S0_MULT_GL) {
     return bloom_test(s.bh, s.bf, key);
   } else {  // S0_GENERIC
     if (s.bh.kind == BLOOM_NONE) return true;
+    if (s.bh.kind == BLOOM_MOD || s.bh.kind == BLOOM_MOD_POW2) {
+      uint32_t bits[3];
+      if (bloom_bits(s.bh, key, bits) == 0) return false;
+      const uint32_t w = bits[0] >> 5;
+      return ((w < s.lds_words ? s.bf[w] : s.gbf[w]) >> (bits[0] & 31)) & 1u;
+    }
     return bloom_test(s.bh, s.bf, key);
   }
 }
@@ -308,11 +419,11 @@ struct Quad {
 
 // ---- group table in LDS --------------------------------------------------
 struct GroupLds {
-  int32_t* key;             // [kGroupSlots]
-  unsigned long long* cnt;  // [kGroupSlots]
-  unsigned long long* sum;  // [kGroupSlots]
-  unsigned long long* prod; // [kGroupSlots]
-  int32_t* pay;             // [kGroupSlots * 4]
+  int32_t* key;             // [slots]
+  unsigned long long* cnt;  // [slots]
+  unsigned long long* sum;  // [slots]
+  unsigned long long* prod; // [slots]
+  int32_t* pay;             // [slots * 4]
 };
 
 __device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, long long s, unsigned long long c,
@@ -381,8 +492,9 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
   if (!alive || g < 0) return n_bad;
   const unsigned long long p =
       a.want_product ? (unsigned long long)(long long)(int32_t)((uint32_t)m * (uint32_t)pay.v[0]) : 0ull;
-  uint32_t slot = ((uint32_t)g * 0x9E3779B1u) >> (32 - kGroupSlotBits);
-  for (int probe = 0; probe < kGroupSlots; probe++) {
+  const uint32_t smask = (1u << a.gslot_bits) - 1u;
+  uint32_t slot = ((uint32_t)g * 0x9E3779B1u) >> (32 - a.gslot_bits);
+  for (int probe = 0; probe < kGroupProbes; probe++) {
     int32_t k = gt.key[slot];
     if (k == kEmptyGroup) {
       const int32_t old = atomicCAS(&gt.key[slot], kEmptyGroup, (int32_t)g);
@@ -399,7 +511,7 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
       if (a.want_product) atomicAdd(&gt.prod[slot], p);
       return n_bad;
     }
-    slot = (slot + 1) & (kGroupSlots - 1);
+    slot = (slot + 1) & smask;
   }
   global_group_add(a, g, (long long)m, 1ull, p, pay, nd);
   return n_bad;
@@ -676,20 +788,21 @@ __device__ __forceinline__ bool process_dense(const ProbeArgs& a, const uint32_t
 #pragma unroll
     for (int j = 0; j < 4; j++) bad |= (uint32_t)(((mask >> j) & 1u) && vget(kv, j) < D.keys_min) << j;
     st.v[ST_OOB] += popc_ballot4(bad);
+    if (LIP) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      if ((mask >> j) & 1u) {
-        const int32_t k = vget(kv, j);
-        key[j].set(d, k);
-        bool ok;
-        if (!LIP) {
+      for (int j = 0; j < 4; j++) key[j].set(d, vget(kv, j));
+      nm = probe_filter4<K0, SPEC>(a, D, lds, kv, mask);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if ((mask >> j) & 1u) {
+          const int32_t k = vget(kv, j);
+          key[j].set(d, k);
           int32_t p = 0;
-          ok = pht_probe(D, k, p);
+          const bool ok = pht_probe(D, k, p);
           pay[j].set(d, p);
-        } else {
-          ok = probe_filter<K0, SPEC>(a, D, lds, k);
+          nm |= (uint32_t)ok << j;
         }
-        nm |= (uint32_t)ok << j;
       }
     }
     mask = nm;
@@ -889,7 +1002,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
     const int32_t km = s0.kmin;
     st.v[ST_OOB] += __popcll(__ballot((valid4 & 1u) && k[v].x < km)) + __popcll(__ballot((valid4 & 2u) && k[v].y < km)) +
                     __popcll(__ballot((valid4 & 4u) && k[v].z < km)) + __popcll(__ballot((valid4 & 8u) && k[v].w < km));
-    mask[v] = s0_probe4<K0>(s0, k[v], valid4, pay[v]);
+    mask[v] = s0_probe4<K0, SPEC != 1>(s0, k[v], valid4, pay[v]);
   }
   const uint32_t nvalid = CHECKED ? (tc.n - base < TILE ? tc.n - base : TILE) : TILE;
   st.v[ST_ROWS_IN + 0] += nvalid;
@@ -1157,15 +1270,16 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   size_t off = ((size_t)a.lds_filter_words * 4 + 15) & ~(size_t)15;
   int32_t* q_base = reinterpret_cast<int32_t*>(lds_raw + off) + wave * (3 * kQueueCap);
   off += (size_t)NW * 3 * kQueueCap * 4;
+  const int gslots = 1 << a.gslot_bits;
   GroupLds gt;
   gt.cnt = reinterpret_cast<unsigned long long*>(lds_raw + off);
-  gt.sum = gt.cnt + kGroupSlots;
-  gt.prod = gt.sum + kGroupSlots;
-  off += 3 * kGroupSlots * 8;
+  gt.sum = gt.cnt + gslots;
+  gt.prod = gt.sum + gslots;
+  off += 3 * (size_t)gslots * 8;
   gt.key = reinterpret_cast<int32_t*>(lds_raw + off);
-  off += kGroupSlots * 4;
+  off += (size_t)gslots * 4;
   gt.pay = reinterpret_cast<int32_t*>(lds_raw + off);
-  off += kGroupSlots * kMaxDims * 4;
+  off += (size_t)gslots * kMaxDims * 4;
   long long* lds_stats = reinterpret_cast<long long*>(lds_raw + off);
   off += kStats * 8;
   unsigned long long* lds_samp = reinterpret_cast<unsigned long long*>(lds_raw + off);  // [kMaxDims + 1]
@@ -1196,7 +1310,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   for (int d = 0; d < a.n_dims; d++) {
     const ProbeDim& D = a.dim[d];
     if (D.bloom_lds >= 0) {
-      const int nw = D.bloom_words;
+      const int nw = D.bloom_lds_words;  // the staged prefix
       uint32_t* dst = lds_filter + D.bloom_lds;
       if (((D.bloom_lds | nw) & 3) == 0 && (((uintptr_t)D.bloom) & 15) == 0) {
         const uint4* src4 = reinterpret_cast<const uint4*>(D.bloom);
@@ -1207,7 +1321,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       }
     }
   }
-  for (int i = threadIdx.x; i < kGroupSlots; i += WG) {
+  for (int i = threadIdx.x; i < gslots; i += WG) {
     gt.key[i] = kEmptyGroup;
     gt.cnt[i] = 0;
     gt.sum[i] = 0;
@@ -1260,6 +1374,8 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   s0.bf = (K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MULT_LDS || (K0 == S0_GENERIC && D0.bloom_lds >= 0))
               ? lds_filter + D0.bloom_lds
               : D0.bloom;
+  s0.gbf = D0.bloom;
+  s0.lds_words = D0.bloom_lds >= 0 ? (uint32_t)D0.bloom_lds_words : 0u;
   s0.ht = D0.ht;
   s0.bh = D0.bh;
   s0.fm = D0.ht_len;
@@ -1335,7 +1451,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kGroupSlots; i += WG) {
+  for (int i = threadIdx.x; i < gslots; i += WG) {
     const int32_t g = gt.key[i];
     if (g != kEmptyGroup && gt.cnt[i] != 0) {
       Quad pay;
@@ -1372,8 +1488,17 @@ template <int WG, int VEC, int K0>
 __device__ __forceinline__ void probe_dispatch(const ProbeArgs& a, uint32_t bid, uint32_t nblk) {
   constexpr bool has_spec = K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_GL ||
                             K0 == S0_MULT_LDS || K0 == S0_MULT_GL;
+  // A filter staged as a prefix with bits set past it (selected dimension
+  // keys outside the table's domain, counted by the build): the generic body,
+  // whose filter reads fall back to the global words past the prefix.
+  bool past = false;
+#pragma unroll
+  for (int d = 0; d < kMaxDims; d++)
+    if (d < a.n_dims && a.dim[d].past_count && a.dim[d].bloom_lds >= 0 &&
+        a.dim[d].bloom_lds_words < a.dim[d].bloom_words)
+      past = past || *a.dim[d].past_count != 0ull;
   if constexpr (has_spec) {
-    if (a.all_k0) probe_body<WG, VEC, K0, 1>(a, bid, nblk);
+    if (a.all_k0 && !past) probe_body<WG, VEC, K0, 1>(a, bid, nblk);
     else probe_body<WG, VEC, K0, 2>(a, bid, nblk);
   } else {
     probe_body<WG, VEC, K0, 2>(a, bid, nblk);

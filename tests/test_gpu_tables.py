@@ -156,3 +156,38 @@ def test_compat_and_textbook_diverge_as_documented(ctx):
     assert out[L.HT_TEXTBOOK]["n_match"] == int(in_dom.sum())
     # compat: every key but the ones on key 0's all-zero slot (fk % nd == 0)
     assert out[L.HT_PHT2]["n_match"] == int((fk % nd != 0).sum())
+
+
+@pytest.mark.parametrize("method", ["lip", "lip-adaptive"])
+def test_filter_bits_past_the_staged_prefix(ctx, method):
+    """A modulo filter is staged in LDS up to the bits the table's key domain
+    can set (ht_len); a dimension key beyond the domain sets a bit past that
+    prefix, and fact keys that reach it read the filter's global words.  The
+    results must still equal the oracle's (the PHT aliases such keys)."""
+    _flush(f"filter prefix {method}")
+    nd, nf = 3000, (1 << 19) + 7
+    rng = np.random.default_rng(61)
+    keys = (rng.permutation(nd) + 1).astype(np.int32)
+    # 40 keys moved beyond the domain by ht_len: the same PHT slot (no
+    # collision: the insert order cannot matter), a filter bit past the
+    # staged prefix (bits 3100..6000 of 8192; the prefix covers 3072)
+    moved = np.nonzero(keys > 100)[0][:40]
+    keys[moved] += nd
+    attr = rng.integers(0, 10, nd).astype(np.int32)
+    k1 = (rng.permutation(2000) + 1).astype(np.int32)
+    a1 = rng.integers(0, 10, 2000).astype(np.int32)
+    fk = rng.integers(1, 8192, nf).astype(np.int32)                # all of the filter's bits are reached
+    f1 = rng.integers(1, 2001, nf).astype(np.int32)
+    meas = rng.integers(1, 1000, nf).astype(np.int32)
+    dk, da, dk1, da1, dfk, df1, dm = (_dev(x) for x in (keys, attr, k1, a1, fk, f1, meas))
+    pred = ((L.COMB_SET, L.CMP_LT, 6),)
+    d0 = L.dim_desc(dk, nd, nd, attr=da, predicate=pred, bloom=("mod", 8192))
+    d1 = L.dim_desc(dk1, 2000, 2000, attr=da1, predicate=pred, bloom=("mod", 2048))
+    q = L.star_desc([dfk, df1], dm, nf, [d0, d1], (0, 1), method=method, group_mult=(10, 1, 0, 0), n_groups=100)
+    res = ctx.star_join(q, groups_cap=256)
+    od = [O.DimSpec(keys, attr, nd, pred=[(O.COMB_SET, O.CMP_LT, 6)], bloom=O.bloom_spec(O.BLOOM_MOD, 8192, 1)),
+          O.DimSpec(k1, a1, 2000, pred=[(O.COMB_SET, O.CMP_LT, 6)], bloom=O.bloom_spec(O.BLOOM_MOD, 2048, 1))]
+    ref, lines, rows, _ = O.star_join(od, [fk, f1], meas, (0, 1), method=O.LIP, group_mult=(10, 1, 0, 0),
+                                      n_groups=100, threads=8)
+    assert _groups(res) == [(g, p, s, c) for g, p, s, c in ref]
+    assert res["stats"]["rows_bloom_pass"] == rows[3]
