@@ -185,6 +185,50 @@ def words_hash(words):
         return "blake2b-" + hashlib.blake2b(words.tobytes(), digest_size=8).hexdigest()
 
 
+def bench_bf_size(ctx, reps=3):
+    """The reference's one logged measurement of this path (eval/plot_bf_size.py:
+    110-115): bin/ops/bloom_join --n=2^30 --d=524288 --s=4096 --b=4096, i.e. a
+    2^30-row fact table over a 524,288-row dimension, dim_val <= 4096 selected,
+    a 4,096-bit modulo filter; time_probe 20.1574 / 20.0898 / 20.098 ms on
+    unstated hardware.  Inputs from the reference's own generator (srand(0),
+    generator.h:37-136) on the host; the same join through lj_hash_join, its
+    counts and sums checked against the oracle."""
+    import torch
+    import lipjoin as L
+    n, d, sel, bits = 1 << 30, 524288, 4096, 4096
+    log(f"bf_size: generating {n} fact rows with the reference generator (host rand())")
+    g = L.gen_reference_host(d, n, want=("dim_keys", "dim_vals", "fact_keys"))
+    dk, dv = torch.from_numpy(g["dim_keys"]).cuda(), torch.from_numpy(g["dim_vals"]).cuda()
+    fk = torch.from_numpy(g["fact_keys"]).cuda()
+    fv = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.gen_device(fv, L.GEN_SEQUENCE, 0, 0, 0, 0)          # dummy_initialize: 0..n-1
+    torch.cuda.synchronize()
+    desc = L.dim_desc(dk, d, d, attr=dv, predicate=((L.COMB_SET, L.CMP_LTE, sel),), bloom=("mod", bits))
+    runs = [ctx.hash_join(fk, fv, n, desc, method="lip") for _ in range(reps + 1)][1:]
+    r = runs[-1]
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    spec = O.DimSpec(g["dim_keys"], g["dim_vals"], d, pred=[(O.COMB_SET, O.CMP_LTE, sel)],
+                     bloom=O.bloom_spec(O.BLOOM_MOD, bits, 1))
+    ref = O.single_join(spec, g["fact_keys"], np.arange(n, dtype=np.int32), method=O.LIP,
+                        threads=host_cpu_share()[0])
+    probe = statistics.median(x["time_probe"] for x in runs)
+    out = {"workload": "bloom_join --n=2^30 --d=524288 --s=4096 --b=4096 (reference generator inputs)",
+           "time_probe_ms": probe, "time_build_ms": statistics.median(x["time_build"] for x in runs),
+           "time_extra_ms": statistics.median(x["time_extra"] for x in runs),
+           "time_join_total_ms": statistics.median(x["time_total"] for x in runs),
+           "fact_rows_per_s": n / (probe * 1e-3), "n_match": r["n_match"], "n_bloom_pass": r["n_bloom_pass"],
+           "reference_logged_time_probe_ms": [20.1574, 20.0898, 20.098],
+           "probe_speedup_vs_reference_log": 20.098 / probe,
+           "matches_oracle": (r["n_bloom_pass"], r["n_match"], r["sum_measure"], r["sum_product"]) ==
+                             (ref["n_bloom_pass"], ref["n_match"], ref["sum_measure"], ref["sum_product"])}
+    log(f"bf_size: probe {probe:.3f} ms (reference log 20.1 ms), matches oracle {out['matches_oracle']}")
+    del g, dk, dv, fk, fv
+    torch.cuda.empty_cache()
+    return out
+
+
 def L_dim(keys, n, attr, pred, bloom):
     import lipjoin as L
     return L.dim_desc(keys, n, n, attr=attr, predicate=pred, bloom=bloom)
@@ -412,6 +456,8 @@ def main():
                     help="config-5 requests per arrival rate (0 skips the serving measurement)")
     ap.add_argument("--no-single", action="store_true", help="skip the config 1/2 single-join measurements")
     ap.add_argument("--no-ssb", action="store_true", help="skip the SSB flight 2-4 query measurements")
+    ap.add_argument("--no-bf-size", action="store_true",
+                    help="skip the reference's logged 2^30-row Bloom-join workload")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -548,6 +594,8 @@ def main():
         out["serving"] = bench_serve(ctx, args.serve_requests)
     if not args.no_single:
         out["single_join"] = bench_single(ctx)
+    if not args.no_bf_size:
+        out["reference_logged_workload"] = bench_bf_size(ctx)
     if not args.no_cpu_baseline:
         threads, visible = host_cpu_share()
         log(f"cpu baseline on {threads} threads")

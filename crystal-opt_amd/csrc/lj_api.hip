@@ -937,7 +937,10 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
       int64_t pw = filter_prefix_words(c, D);
       if (pw < D->bloom_words) {
         if (q->resident[d] && D->n_past != 0) pw = D->bloom_words;
-        else if (!q->resident[d]) P.past_count = D->counters + kCounterPast;
+        else if (!q->resident[d]) {
+          P.past_count = D->counters + kCounterPast;
+          a.check_past = 1;
+        }
       }
       P.bloom_lds_words = (int32_t)pw;
     }
@@ -1428,6 +1431,45 @@ extern "C" lj_status lj_gen_shuffle_host(uint64_t seed, uint64_t stream, int64_t
     out[i] = out[j];
     out[j] = t;
   }
+  return LJ_OK;
+}
+
+// RAND_RANGE(N) of generator.h:32 over the C library's rand(), truncated to
+// int as `int j = RAND_RANGE(i)` does (generator.h:56).
+static int32_t ref_rand_range(int64_t n) { return (int32_t)((double)rand() / ((double)RAND_MAX + 1) * (double)n); }
+
+// random_unique_gen (generator.h:69-79): 1..n, then knuth_shuffle (:51-60).
+static void ref_unique_shuffled(int32_t* a, int64_t n, std::vector<int32_t>& scratch) {
+  int32_t* out = a;
+  if (!out) {
+    scratch.resize((size_t)n);
+    out = scratch.data();
+  }
+  for (int64_t i = 0; i < n; i++) out[i] = (int32_t)(i + 1);
+  for (int64_t i = n - 1; i > 0; i--) {
+    const int32_t j = ref_rand_range(i);
+    const int32_t t = out[i];
+    out[i] = out[j];
+    out[j] = t;
+  }
+}
+
+extern "C" lj_status lj_gen_reference_host(int64_t n_dim, int64_t n_fact, int32_t* dim_keys, int32_t* dim_vals,
+                                           int32_t* fact_keys, int32_t* fact_vals) {
+  if (n_dim < 1 || n_dim >= (1ll << 31) || n_fact < 0 || n_fact >= (1ll << 31))
+    return fail(nullptr, LJ_EINVAL, "n_dim %lld / n_fact %lld out of range", (long long)n_dim, (long long)n_fact);
+  std::vector<int32_t> scratch;
+  srand(0);  // check_seed (generator.h:38-45)
+  // create_relation_pk (generator.h:88-106)
+  ref_unique_shuffled(dim_keys, n_dim, scratch);
+  if (dim_vals)
+    for (int64_t i = 0; i < n_dim; i++) dim_vals[i] = (int32_t)i;
+  // create_relation_fk (generator.h:108-136)
+  const int64_t iters = n_fact / n_dim, rem = n_fact % n_dim;
+  for (int64_t i = 0; i < iters; i++) ref_unique_shuffled(fact_keys ? fact_keys + n_dim * i : nullptr, n_dim, scratch);
+  if (rem > 0) ref_unique_shuffled(fact_keys ? fact_keys + n_dim * iters : nullptr, rem, scratch);
+  if (fact_vals)
+    for (int64_t i = 0; i < n_fact; i++) fact_vals[i] = (int32_t)i;
   return LJ_OK;
 }
 

@@ -126,7 +126,7 @@ struct ProbeArgs {
   uint32_t* gseen;
   int64_t g_stride;
   int32_t g_reps;
-  int32_t pad2;
+  int32_t check_past;      // some dimension passes past_count: the probe reads it at its start
   // if set (a single query): the header and the first host_groups groups are
   // also written to this pinned host memory, so no copy follows the probe
   long long* host_hdr;

@@ -6,12 +6,14 @@
 
 namespace lj {
 
+// The probe's LDS layout (probe_body): wave queues, statistics, sample
+// totals, tile counter, the group table, then the staged filters.
 size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words, int group_slots) {
-  size_t off = ((size_t)lds_filter_words * 4 + 15) & ~(size_t)15;
-  off += (size_t)(c.wg / 64) * 3 * kQueueCap * 4;
+  size_t off = (size_t)(c.wg / 64) * 3 * kQueueCap * 4;
+  off += kStats * 8 + (kMaxDims + 1) * 8 + 16;
   off += (size_t)group_slots * kGroupSlotBytes;
-  off += kStats * 8 + (kMaxDims + 1) * 8;
-  off += 16;  // tile counter
+  off = (off + 15) & ~(size_t)15;
+  off += (size_t)lds_filter_words * 4;
   return off;
 }
 

@@ -109,6 +109,7 @@ template <bool FB>
 __device__ __forceinline__ bool staged_bit(const ProbeDim& D, const uint32_t* lds, uint32_t h) {
   const uint32_t P = (uint32_t)D.bloom_lds_words, w = h >> 5;
   const LdsWords f = (LdsWords)(lds + D.bloom_lds);
+  if (P == (uint32_t)D.bloom_words) return (f[w] >> (h & 31)) & 1u;  // the whole filter (uniform)
   if constexpr (FB) {
     return ((w < P ? f[w] : ((GlobalWords)D.bloom)[w]) >> (h & 31)) & 1u;
   } else {
@@ -122,6 +123,13 @@ __device__ __forceinline__ uint32_t staged_bits4(const ProbeDim& D, const uint32
   const uint32_t P = (uint32_t)D.bloom_lds_words;
   const LdsWords f = (LdsWords)(lds + D.bloom_lds);
   uint32_t w[4], m = 0;
+  if (P == (uint32_t)D.bloom_words) {  // the whole filter (uniform)
+#pragma unroll
+    for (int j = 0; j < 4; j++) w[j] = f[h[j] >> 5];
+#pragma unroll
+    for (int j = 0; j < 4; j++) m |= __builtin_amdgcn_ubfe(w[j], h[j], 1u) << j;
+    return m;
+  }
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const uint32_t wi = h[j] >> 5;
@@ -269,6 +277,7 @@ struct S0 {
   const uint32_t* bf;    // staged copy in LDS or the global words
   const uint32_t* gbf;   // the global words
   uint32_t lds_words;    // words of bf that are staged (a prefix); the rest is read from gbf
+  bool whole;            // the whole filter is staged
   const int32_t* ht;
   BloomHash bh;
   FastMod fm;
@@ -311,8 +320,15 @@ __device__ __forceinline__ uint32_t s0_probe4(const S0& s, const v4i32& k, uint3
     // v_bfe_u32 reads bits [4:0] of the offset: bit h & 31 of the word
     if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS) {
       // the staged prefix (see staged_bit): FB reads a word past it from the
-      // global words; otherwise such a bit is 0
+      // global words; otherwise such a bit is 0.  The whole filter staged (a
+      // uniform test): plain LDS reads.
       const uint32_t P = s.lds_words;
+      if (s.whole) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) wv[j] = ((LdsWords)s.bf)[wi[j]];
+#pragma unroll
+        for (int j = 0; j < 4; j++) ok |= __builtin_amdgcn_ubfe(wv[j], h[j], 1u) << j;
+      } else {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if constexpr (FB) wv[j] = wi[j] < P ? ((LdsWords)s.bf)[wi[j]] : ((GlobalWords)s.gbf)[wi[j]];
@@ -323,6 +339,7 @@ __device__ __forceinline__ uint32_t s0_probe4(const S0& s, const v4i32& k, uint3
         uint32_t b = __builtin_amdgcn_ubfe(wv[j], h[j], 1u);
         if constexpr (!FB) b &= (uint32_t)(wi[j] < P);
         ok |= b << j;
+      }
       }
     } else {
 #pragma unroll
@@ -1266,10 +1283,15 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   const int wave = uniform(threadIdx.x >> 6);  // known wave-uniform: keeps per-wave branches scalar
   const int lane = (int)lane_id();
 
-  uint32_t* lds_filter = reinterpret_cast<uint32_t*>(lds_raw);
-  size_t off = ((size_t)a.lds_filter_words * 4 + 15) & ~(size_t)15;
-  int32_t* q_base = reinterpret_cast<int32_t*>(lds_raw + off) + wave * (3 * kQueueCap);
-  off += (size_t)NW * 3 * kQueueCap * 4;
+  // LDS layout: the fixed-size parts first, at constant offsets (the wave
+  // queues, statistics, sample totals, tile counter), then the group table
+  // (its size is the launch's), then the staged filters (probe_lds_bytes).
+  constexpr size_t kQueueBytes = (size_t)NW * 3 * kQueueCap * 4;
+  int32_t* q_base = reinterpret_cast<int32_t*>(lds_raw) + wave * (3 * kQueueCap);
+  long long* lds_stats = reinterpret_cast<long long*>(lds_raw + kQueueBytes);
+  unsigned long long* lds_samp = reinterpret_cast<unsigned long long*>(lds_raw + kQueueBytes + kStats * 8);  // [kMaxDims + 1]
+  uint32_t* lds_next = reinterpret_cast<uint32_t*>(lds_raw + kQueueBytes + kStats * 8 + (kMaxDims + 1) * 8);  // tile counter
+  size_t off = kQueueBytes + kStats * 8 + (kMaxDims + 1) * 8 + 16;
   const int gslots = 1 << a.gslot_bits;
   GroupLds gt;
   gt.cnt = reinterpret_cast<unsigned long long*>(lds_raw + off);
@@ -1280,11 +1302,8 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   off += (size_t)gslots * 4;
   gt.pay = reinterpret_cast<int32_t*>(lds_raw + off);
   off += (size_t)gslots * kMaxDims * 4;
-  long long* lds_stats = reinterpret_cast<long long*>(lds_raw + off);
-  off += kStats * 8;
-  unsigned long long* lds_samp = reinterpret_cast<unsigned long long*>(lds_raw + off);  // [kMaxDims + 1]
-  off += (kMaxDims + 1) * 8;
-  uint32_t* lds_next = reinterpret_cast<uint32_t*>(lds_raw + off);  // tile counter
+  off = (off + 15) & ~(size_t)15;
+  uint32_t* lds_filter = reinterpret_cast<uint32_t*>(lds_raw + off);
 
   if (bid == 0 && threadIdx.x == 0) a.out_hdr[kClockSlot + 1] = (long long)wall_clock64();
 
@@ -1376,6 +1395,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
               : D0.bloom;
   s0.gbf = D0.bloom;
   s0.lds_words = D0.bloom_lds >= 0 ? (uint32_t)D0.bloom_lds_words : 0u;
+  s0.whole = D0.bloom_lds >= 0 && D0.bloom_lds_words == D0.bloom_words;
   s0.ht = D0.ht;
   s0.bh = D0.bh;
   s0.fm = D0.ht_len;
@@ -1492,11 +1512,13 @@ __device__ __forceinline__ void probe_dispatch(const ProbeArgs& a, uint32_t bid,
   // keys outside the table's domain, counted by the build): the generic body,
   // whose filter reads fall back to the global words past the prefix.
   bool past = false;
+  if (a.check_past) {
 #pragma unroll
-  for (int d = 0; d < kMaxDims; d++)
-    if (d < a.n_dims && a.dim[d].past_count && a.dim[d].bloom_lds >= 0 &&
-        a.dim[d].bloom_lds_words < a.dim[d].bloom_words)
-      past = past || *a.dim[d].past_count != 0ull;
+    for (int d = 0; d < kMaxDims; d++)
+      if (d < a.n_dims && a.dim[d].past_count && a.dim[d].bloom_lds >= 0 &&
+          a.dim[d].bloom_lds_words < a.dim[d].bloom_words)
+        past = past || *a.dim[d].past_count != 0ull;
+  }
   if constexpr (has_spec) {
     if (a.all_k0 && !past) probe_body<WG, VEC, K0, 1>(a, bid, nblk);
     else probe_body<WG, VEC, K0, 2>(a, bid, nblk);

@@ -45,19 +45,6 @@ static const char* arg_str(int argc, char** argv, const char* name) {
   return nullptr;
 }
 
-// RAND_RANGE(N) of generator.h:32 over the C library's rand().
-static int rand_range(int n) { return (int)((double)rand() / ((double)RAND_MAX + 1) * n); }
-
-static void shuffle_unique(int* a, int n) {  // values 1..n, then the reference's shuffle
-  for (int i = 0; i < n; i++) a[i] = i + 1;
-  for (int i = n - 1; i > 0; i--) {
-    int j = rand_range(i);
-    int t = a[i];
-    a[i] = a[j];
-    a[j] = t;
-  }
-}
-
 #define CHECK_LJ(ctx, x)                                                                   \
   do {                                                                                     \
     lj_status s_ = (x);                                                                    \
@@ -107,18 +94,11 @@ int main(int argc, char** argv) {
   CHECK_HIP(hipMalloc(&d_fact_fkey, sizeof(int32_t) * num_fact));
   CHECK_HIP(hipMalloc(&d_fact_val, sizeof(int32_t) * num_fact));
   if (!seeded) {
-    // create_relation_pk / create_relation_fk, srand(0)
-    srand(0);
-    std::vector<int32_t> k(num_dim), v(num_dim);
-    shuffle_unique(k.data(), (int)num_dim);
-    for (long long i = 0; i < num_dim; i++) v[i] = (int32_t)i;
+    // create_relation_pk / create_relation_fk after srand(0) (generator.h:37-136)
+    std::vector<int32_t> k(num_dim), v(num_dim), fk(num_fact), fv(num_fact);
+    CHECK_LJ(nullptr, lj_gen_reference_host(num_dim, num_fact, k.data(), v.data(), fk.data(), fv.data()));
     CHECK_HIP(hipMemcpy(d_dim_key, k.data(), 4 * num_dim, hipMemcpyHostToDevice));
     CHECK_HIP(hipMemcpy(d_dim_val, v.data(), 4 * num_dim, hipMemcpyHostToDevice));
-    std::vector<int32_t> fk(num_fact), fv(num_fact);
-    const long long iters = num_fact / num_dim, rem = num_fact % num_dim;
-    for (long long i = 0; i < iters; i++) shuffle_unique(fk.data() + num_dim * i, (int)num_dim);
-    if (rem > 0) shuffle_unique(fk.data() + num_dim * iters, (int)rem);
-    for (long long i = 0; i < num_fact; i++) fv[i] = (int32_t)i;
     CHECK_HIP(hipMemcpy(d_fact_fkey, fk.data(), 4 * num_fact, hipMemcpyHostToDevice));
     CHECK_HIP(hipMemcpy(d_fact_val, fv.data(), 4 * num_fact, hipMemcpyHostToDevice));
   } else {

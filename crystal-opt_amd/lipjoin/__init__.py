@@ -382,6 +382,20 @@ def gen_host(kind, seed, stream, lo, hi, n, first=0):
     return out
 
 
+def gen_reference_host(n_dim, n_fact, want=("dim_keys", "dim_vals", "fact_keys", "fact_vals")):
+    """The reference's single-join inputs (generator.h:37-136 after srand(0)):
+    a dict of the numpy int32 columns named in `want`."""
+    import numpy as np
+    names = ("dim_keys", "dim_vals", "fact_keys", "fact_vals")
+    sizes = (n_dim, n_dim, n_fact, n_fact)
+    out = {k: np.empty(int(n), dtype=np.int32) for k, n in zip(names, sizes) if k in want}
+    ptr = [out[k].ctypes.data if k in out and out[k].size else None for k in names]
+    st = lib().lj_gen_reference_host(int(n_dim), int(n_fact), *ptr)
+    if st != _abi.OK:
+        raise LJError(st, lib().lj_last_error(None).decode())
+    return out
+
+
 def shuffle_host(arr, seed, stream):
     """In-place Fisher-Yates shuffle of an int32 numpy array."""
     st = lib().lj_gen_shuffle_host(seed, stream, arr.size, arr.ctypes.data if arr.size else None)

@@ -407,6 +407,17 @@ lj_status lj_gen_device(lj_ctx* ctx, int32_t kind, uint64_t seed, uint64_t strea
 /* Host Fisher-Yates shuffle of out[0..n) driven by the same generator. */
 lj_status lj_gen_shuffle_host(uint64_t seed, uint64_t stream, int64_t n, int32_t* out);
 
+/* The reference's single-join inputs (src/ops/utils/generator.h:37-136, as
+ * bloom_join.cu:221-222 calls them): srand(0), then create_relation_pk --
+ * dimension keys 1..n_dim shuffled by its Knuth shuffle (j = RAND_RANGE(i),
+ * generator.h:32,51-60), values 0..n_dim-1 -- then create_relation_fk -- fact
+ * keys as successive shuffled blocks of 1..n_dim and a shuffled 1..(n_fact %
+ * n_dim) tail, values 0..n_fact-1.  Uses the C library's rand() (its global
+ * state: not thread-safe).  Any output pointer may be NULL (its values are
+ * still drawn, so the others do not change). */
+lj_status lj_gen_reference_host(int64_t n_dim, int64_t n_fact, int32_t* dim_keys, int32_t* dim_vals,
+                                int32_t* fact_keys, int32_t* fact_vals);
+
 #ifdef __cplusplus
 }
 #endif
