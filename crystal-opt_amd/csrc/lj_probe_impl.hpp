@@ -766,110 +766,47 @@ __device__ __forceinline__ void rotate_batch(const ProbeArgs& a, const uint32_t*
   w.qcount = rest;
 }
 
-// In-place processing of one segment (lane: rows row0..row0+3).  key0 / pay0:
-// stage-0 keys and payloads; mask: alive bits.  If use_eager, `eager` holds
-// the lane's 4 values of the next column already (stage 1's FK column, or the
-// measure column when there is a single dimension).  With two or more
-// dimensions only stages 0 and 1 run here: the rest is left to the wave queue
-// (batches of 64 rows, so the later gathers of one segment do not each wait a
-// round trip): returns true with the stage-1 survivors in qmask and their
-// stage-0 / stage-1 keys (LIP) or payloads (naive) in qx0 / qx1.  With one
-// dimension the segment is finished here (returns false).
+// Stage 1 of a dense segment, in place (lane: rows row0..row0+3; two or
+// more dimensions).  key0 / pay0: the stage-0 keys and payloads; mask: the
+// stage-0 survivors.  If use_eager, `eager` holds the lane's 4 keys of stage
+// 1's FK column already (streamed with stage 0's).  Later stages are left to
+// the wave queue (batches of 64 rows, so the later gathers of one segment do
+// not each wait a round trip).  Returns the stage-1 survivors, with their
+// stage-0 / stage-1 keys (LIP) or payloads (naive) in qx0 / qx1: the two
+// columns' registers themselves, so no per-dimension register set is kept.
 template <int K0, int SPEC>
-__device__ __forceinline__ bool process_dense(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
-                                              uint32_t row0, const v4i32& key0, const int32_t (&pay0)[4],
-                                              uint32_t mask, const v4i32& eager, bool use_eager, WaveStats& st,
-                                              uint32_t& qmask, int32_t (&qx0)[4], int32_t (&qx1)[4]) {
+__device__ __forceinline__ uint32_t dense_stage1(const ProbeArgs& a, const uint32_t* lds, const WaveCtx& w,
+                                                 uint32_t row0, const v4i32& key0, const int32_t (&pay0)[4],
+                                                 uint32_t mask, const v4i32& eager, bool use_eager, WaveStats& st,
+                                                 int32_t (&qx0)[4], int32_t (&qx1)[4]) {
   constexpr bool LIP = K0 != S0_PHT;
-  const int nd = w.nd;
-  const int64_t n = a.n;
-  Quad key[4], pay[4];
+  const ProbeDim& D = a.dim[w.ld1];
+  st.v[ST_LINES_IN + 1] += lines_of_ballot(__ballot(mask != 0));
+  st.v[ST_ROWS_IN + 1] += popc_ballot4(mask);
+  const v4i32 kv = use_eager ? eager : WavePredLoadVec4(gl(D.fk), row0, a.n, mask);
+  const int32_t kmin = D.keys_min;
+  uint32_t bad = 0;
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    key[j].clear();
-    pay[j].clear();
-    key[j].set(w.ld0, vget(key0, j));
-    pay[j].set(w.ld0, pay0[j]);
-  }
-#pragma unroll
-  for (int s = 1; s < kMaxDims; s++) {
-    if (s >= nd) continue;
-    const int d = w.ld(s);
-    const ProbeDim& D = a.dim[d];
-    st.v[ST_LINES_IN + s] += lines_of_ballot(__ballot(mask != 0));
-    st.v[ST_ROWS_IN + s] += popc_ballot4(mask);
-    v4i32 kv;
-    if (s == 1 && use_eager) kv = eager;
-    else kv = WavePredLoadVec4(gl(D.fk), row0, n, mask);
-    uint32_t nm = 0, bad = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) bad |= (uint32_t)(((mask >> j) & 1u) && vget(kv, j) < D.keys_min) << j;
-    st.v[ST_OOB] += popc_ballot4(bad);
-    if (LIP) {
-#pragma unroll
-      for (int j = 0; j < 4; j++) key[j].set(d, vget(kv, j));
-      nm = probe_filter4<K0, SPEC>(a, D, lds, kv, mask);
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        if ((mask >> j) & 1u) {
-          const int32_t k = vget(kv, j);
-          key[j].set(d, k);
-          int32_t p = 0;
-          const bool ok = pht_probe(D, k, p);
-          pay[j].set(d, p);
-          nm |= (uint32_t)ok << j;
-        }
-      }
-    }
-    mask = nm;
-    if (s == 1) {
-      qmask = mask;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        qx0[j] = LIP ? key[j].get(w.ld0) : pay[j].get(w.ld0);
-        qx1[j] = LIP ? key[j].get(d) : pay[j].get(d);
-      }
-      return true;
-    }
-  }
+  for (int j = 0; j < 4; j++) bad |= (uint32_t)(vget(kv, j) < kmin) << j;
+  st.v[ST_OOB] += popc_ballot4(bad & mask);
+  uint32_t nm = 0;
   if (LIP) {
-    st.v[ST_BLOOM_PASS] += popc_ballot4(mask);
+    nm = probe_filter4<K0, SPEC>(a, D, lds, kv, mask);
 #pragma unroll
-    for (int t = 0; t < kMaxDims; t++) {
-      if (t >= nd) continue;
-      const int e = a.order[t];
-      uint32_t nm = 0;
+    for (int j = 0; j < 4; j++) {
+      qx0[j] = vget(key0, j);
+      qx1[j] = vget(kv, j);
+    }
+  } else {
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        if ((mask >> j) & 1u) {
-          int32_t p = 0;
-          const bool ok = pht_probe(a.dim[e], key[j].get(e), p);
-          pay[j].set(e, p);
-          nm |= (uint32_t)ok << j;
-        }
-      }
-      mask = nm;
+    for (int j = 0; j < 4; j++) {
+      int32_t p = 0;
+      if ((mask >> j) & 1u) nm |= (uint32_t)pht_probe(D, vget(kv, j), p) << j;
+      qx0[j] = pay0[j];
+      qx1[j] = p;
     }
   }
-  st.meas_lines += lines_of_ballot(__ballot(mask != 0));
-  st.meas_rows += popc_ballot4(mask);
-  v4i32 mv;
-  if (nd == 1 && use_eager) mv = eager;
-  else {
-    mv = WavePredLoadVec4(gl(a.measure), row0, n, mask);
-    if (a.measure_sub) {
-      // Q4.x: revenue - supplycost, int32 wrap-around as the reference's int subtraction
-      const v4i32 sv = WavePredLoadVec4(gl(a.measure_sub), row0, n, mask);
-      mv.x = (int32_t)((uint32_t)mv.x - (uint32_t)sv.x);
-      mv.y = (int32_t)((uint32_t)mv.y - (uint32_t)sv.y);
-      mv.z = (int32_t)((uint32_t)mv.z - (uint32_t)sv.z);
-      mv.w = (int32_t)((uint32_t)mv.w - (uint32_t)sv.w);
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < 4; j++) st.v[ST_OOB] += group_add(a, gt, (mask >> j) & 1u, pay[j], vget(mv, j));
-  return false;
+  return nm;
 }
 
 // Append the alive rows of one segment to the wave queue (row order kept).
@@ -1063,8 +1000,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
 #pragma unroll
         for (int j = 0; j < 4; j++) qx1[j] = vget(ev, j);
       }
-      if (w.nd >= 2 && !process_dense<K0, SPEC>(a, lds, gt, w, row0, kv, pv, m, ev, have_eager, st, qm, qx0, qx1))
-        continue;
+      if (w.nd >= 2) qm = dense_stage1<K0, SPEC>(a, lds, w, row0, kv, pv, m, ev, have_eager, st, qx0, qx1);
       const int cq = popc_ballot4(qm);
       if (cq == 0) continue;
       // the stage-1 survivors (up to 4 per lane) go in up to four parts of
