@@ -443,8 +443,10 @@ struct GroupLds {
   int32_t* pay;             // [slots * 4]
 };
 
+// pay.v[s] is the payload of dimension dm.v[s] (a stage's dimension, or the
+// identity map).
 __device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, long long s, unsigned long long c,
-                                                 unsigned long long p, const Quad& pay, int nd) {
+                                                 unsigned long long p, const Quad& pay, const Quad& dm, int nd) {
   if (a.g_reps > 1) {
     // this workgroup's copy; the first add of a group to any copy lists it once
     const int64_t gi = g + (int64_t)(blockIdx.x & (uint32_t)(a.g_reps - 1)) * a.g_stride;
@@ -453,8 +455,8 @@ __device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, 
     const unsigned long long old = atomicAdd(&a.gcount[gi], c);
     if (old == 0 && atomicExch(&a.gseen[g], 1u) == 0u) {
 #pragma unroll
-      for (int d = 0; d < kMaxDims; d++)
-        if (d < nd) a.gpay[g * kMaxDims + d] = pay.v[d];
+      for (int t = 0; t < kMaxDims; t++)
+        if (t < nd) a.gpay[g * kMaxDims + dm.v[t]] = pay.v[t];
       uint32_t idx = atomicAdd(a.n_touched, 1u);
       if (idx < a.touched_cap) a.touched[idx] = (uint32_t)g;
     }
@@ -465,8 +467,8 @@ __device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, 
   unsigned long long old = atomicAdd(&a.gcount[g], c);
   if (old == 0) {
 #pragma unroll
-    for (int d = 0; d < kMaxDims; d++)
-      if (d < nd) a.gpay[g * kMaxDims + d] = pay.v[d];
+    for (int t = 0; t < kMaxDims; t++)
+      if (t < nd) a.gpay[g * kMaxDims + dm.v[t]] = pay.v[t];
     uint32_t idx = atomicAdd(a.n_touched, 1u);
     if (idx < a.touched_cap) a.touched[idx] = (uint32_t)g;
   }
@@ -474,14 +476,18 @@ __device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, 
 
 // Add one row per alive lane to the workgroup's group table.  Called by every
 // lane of the wave; returns the number of rows (wave-wide) whose group index
-// fell outside [0, n_groups).
-__device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool alive, const Quad& pay, int32_t m) {
+// fell outside [0, n_groups).  pay.v[s] is the payload of stage s's dimension
+// dm.v[s] (wave-uniform), so no register is selected by a dimension index.
+__device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool alive, const Quad& pay,
+                                         const Quad& dm, int32_t m) {
   const int nd = a.n_dims;
+  // the product's payload is dimension 0's (single joins: stage 0)
+  const int32_t pay0 = dm.v[0] == 0 ? pay.v[0] : dm.v[1] == 0 ? pay.v[1] : dm.v[2] == 0 ? pay.v[2] : pay.v[3];
   if (a.n_groups == 1) {
     // one group (single join): wave-reduce, one LDS atomic per wave
     long long s = alive ? (long long)m : 0;
     unsigned long long p = 0;
-    if (a.want_product && alive) p = (unsigned long long)(long long)(int32_t)((uint32_t)m * (uint32_t)pay.v[0]);
+    if (a.want_product && alive) p = (unsigned long long)(long long)(int32_t)((uint32_t)m * (uint32_t)pay0);
     const unsigned long long b = __ballot(alive);
     if (b == 0) return 0;
     s = WaveSum(s);
@@ -492,7 +498,8 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
       if (a.want_product) atomicAdd(&gt.prod[0], p);
       if (gt.key[0] == kEmptyGroup) {
         gt.key[0] = 0;
-        for (int d = 0; d < kMaxDims; d++) gt.pay[d] = pay.v[d];
+        for (int t = 0; t < kMaxDims; t++)
+          if (t < nd) gt.pay[dm.v[t]] = pay.v[t];
       }
     }
     return 0;
@@ -501,14 +508,14 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
   if (alive) {
     int64_t acc = a.goffset;
 #pragma unroll
-    for (int d = 0; d < kMaxDims; d++)
-      if (d < nd) acc += (int64_t)pay.v[d] * a.dim[d].gmult;
+    for (int t = 0; t < kMaxDims; t++)
+      if (t < nd) acc += (int64_t)pay.v[t] * a.dim[dm.v[t]].gmult;
     g = acc % a.n_groups;
   }
   const int n_bad = __popcll(__ballot(alive && g < 0));
   if (!alive || g < 0) return n_bad;
   const unsigned long long p =
-      a.want_product ? (unsigned long long)(long long)(int32_t)((uint32_t)m * (uint32_t)pay.v[0]) : 0ull;
+      a.want_product ? (unsigned long long)(long long)(int32_t)((uint32_t)m * (uint32_t)pay0) : 0ull;
   const uint32_t smask = (1u << a.gslot_bits) - 1u;
   uint32_t slot = ((uint32_t)g * 0x9E3779B1u) >> (32 - a.gslot_bits);
   for (int probe = 0; probe < kGroupProbes; probe++) {
@@ -516,7 +523,8 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
     if (k == kEmptyGroup) {
       const int32_t old = atomicCAS(&gt.key[slot], kEmptyGroup, (int32_t)g);
       if (old == kEmptyGroup) {
-        for (int d = 0; d < kMaxDims; d++) gt.pay[slot * kMaxDims + d] = pay.v[d];
+        for (int t = 0; t < kMaxDims; t++)
+          if (t < nd) gt.pay[slot * kMaxDims + dm.v[t]] = pay.v[t];
         k = (int32_t)g;
       } else {
         k = old;
@@ -530,7 +538,7 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
     }
     slot = (slot + 1) & smask;
   }
-  global_group_add(a, g, (long long)m, 1ull, p, pay, nd);
+  global_group_add(a, g, (long long)m, 1ull, p, pay, dm, nd);
   return n_bad;
 }
 
@@ -640,15 +648,21 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
   const bool done1 = alive && w.p_row < 0;
   const int32_t row = w.p_row & 0x7FFFFFFF;
   const int first = done1 ? 2 : 1;
-  Quad key, pay;
+  // keys and payloads by stage (stage s reads dimension w.ld(s)): registers
+  // picked by compile-time stage numbers only
+  Quad key, pay, dm;
   key.clear();
   pay.clear();
+  dm.v[0] = w.ld0;
+  dm.v[1] = w.ld1;
+  dm.v[2] = w.ld2;
+  dm.v[3] = w.ld3;
   if (alive) {
-    if (LIP) key.set(w.ld0, w.p_x0);
-    else pay.set(w.ld0, w.p_x0);
+    if (LIP) key.v[0] = w.p_x0;
+    else pay.v[0] = w.p_x0;
     if (done1 && w.nd >= 2) {
-      if (LIP) key.set(w.ld1, w.p_x1);
-      else pay.set(w.ld1, w.p_x1);
+      if (LIP) key.v[1] = w.p_x1;
+      else pay.v[1] = w.p_x1;
     }
   }
   const int nd = w.nd;
@@ -666,11 +680,11 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
     if (load) k = s == first ? w.p_k : gl(D.fk)[(uint32_t)row];
     st.v[ST_OOB] += __popcll(__ballot(load && k < D.keys_min));
     if (load) {
-      key.set(d, k);
+      key.v[s] = k;
       if (!LIP) {
         int32_t p = 0;
         alive = pht_probe(D, k, p);
-        pay.set(d, p);
+        pay.v[s] = p;
       } else {
         alive = probe_filter<K0, SPEC>(a, D, lds, k);
       }
@@ -692,7 +706,7 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
         alive = alive && w.p_ht != 0;
         p = (int32_t)(uint32_t)(w.p_ht >> 32);
       }
-      pay.set(0, p);
+      pay.v[0] = p;
       m = w.p_m;
     } else if (bloom_pass != 0) {
       // After the Bloom chain every key is known: the PHT_2 probes of all
@@ -701,16 +715,16 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
       // row, or a key outside the table, reads slot 0 / row 0 and drops it),
       // so no branch separates them.  A row is alive iff every probe hits, as
       // with the probes taken one after another in join order.
-      // (every slot is loaded, the one of a dimension past nd from dimension
-      // 0's table: no branch on nd between the loads either)
+      // (every stage's slot is loaded, a stage past nd from stage 0's table:
+      // no branch on nd between the loads either)
       uint64_t slot[kMaxDims];
       int64_t sls[kMaxDims];
       bool hit = alive;
 #pragma unroll
       for (int t = 0; t < kMaxDims; t++) {
         const bool on = t < nd;
-        const ProbeDim& D = a.dim[on ? t : 0];
-        const int64_t sl = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, key.get(t));
+        const ProbeDim& D = a.dim[on ? dm.v[t] : w.ld0];
+        const int64_t sl = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, key.v[t]);
         hit = hit && (!on || sl >= 0);
         sls[t] = sl;
         slot[t] = ht_load(D.ht_kind, D.ht, alive && on && sl >= 0 ? sl : 0);
@@ -720,15 +734,15 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
 #pragma unroll
       for (int t = 0; t < kMaxDims; t++) {
         if (t >= nd) continue;
-        const ProbeDim& D = a.dim[t];
+        const ProbeDim& D = a.dim[dm.v[t]];
         int32_t p = 0;
         if (D.ht_kind == HT_TEXTBOOK) {
-          if (hit) hit = ht_resolve(D.ht_kind, D.ht, D.ht_len, key.get(t), sls[t], slot[t], p);
+          if (hit) hit = ht_resolve(D.ht_kind, D.ht, D.ht_len, key.v[t], sls[t], slot[t], p);
         } else {
           hit = hit && slot[t] != 0;
           p = (int32_t)(uint32_t)(slot[t] >> 32);
         }
-        pay.set(t, p);
+        pay.v[t] = p;
       }
       alive = hit;
     }
@@ -739,7 +753,7 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
     m = nd == 1 ? w.p_m : a.measure[(uint32_t)row];
     if (nd > 1 && a.measure_sub) m = (int32_t)((uint32_t)m - (uint32_t)a.measure_sub[(uint32_t)row]);
   }
-  st.v[ST_OOB] += group_add(a, gt, alive, pay, m);
+  st.v[ST_OOB] += group_add(a, gt, alive, pay, dm, m);
   w.p_cnt = 0;
 }
 
@@ -1410,9 +1424,12 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   for (int i = threadIdx.x; i < gslots; i += WG) {
     const int32_t g = gt.key[i];
     if (g != kEmptyGroup && gt.cnt[i] != 0) {
-      Quad pay;
-      for (int d = 0; d < kMaxDims; d++) pay.v[d] = gt.pay[i * kMaxDims + d];
-      global_group_add(a, g, (long long)gt.sum[i], gt.cnt[i], gt.prod[i], pay, a.n_dims);
+      Quad pay, ident;
+      for (int d = 0; d < kMaxDims; d++) {
+        pay.v[d] = gt.pay[i * kMaxDims + d];
+        ident.v[d] = d;
+      }
+      global_group_add(a, g, (long long)gt.sum[i], gt.cnt[i], gt.prod[i], pay, ident, a.n_dims);
     }
   }
   for (int i = threadIdx.x; i < kStats; i += WG) a.stats[(int64_t)bid * kStats + i] = lds_stats[i];
