@@ -60,7 +60,7 @@ def bench_batch(ctx, n_inst, reps):
     cols, descs, choices = [], [], []
     for sp in specs:
         c = W.batch_instance_device(ctx, sp)
-        bloom, ch = W.tuned_bloom(ctx, sp)
+        bloom, ch = W.tuned_bloom(ctx, sp, semi_join=True)   # one group, no product
         cols.append(c)
         choices.append(ch)
         descs.append(W.batch_desc(c, sp, method="lip", bloom=bloom))
@@ -83,8 +83,8 @@ def bench_batch(ctx, n_inst, reps):
                      for sp, ch in zip(specs, choices))
     fact_bytes = sum(4 * sp["n_fact"] + 128 * r["stats"]["lines_in"][1] for sp, r in zip(specs, sep))
     out["bytes_alg"] = fact_bytes + dim_bytes
-    for mode, graph in (("eager", False), ("graph", True)):
-        b = ctx.batch(descs, use_graph=graph)
+    for mode, graph, per_kind in (("eager", False, False), ("graph", True, False), ("eager_per_kind", False, True)):
+        b = ctx.batch(descs, use_graph=graph, launch_per_kind=per_kind)
         res = b.run()
         exact = all((r["sum_total"], r["n_rows"]) == (s["sum_total"], s["n_rows"]) for r, s in zip(res, sep))
         t = []
@@ -109,7 +109,8 @@ def bench_batch(ctx, n_inst, reps):
     out["bloom_choices"] = kinds
     out["separate_rows_per_s"] = rows / sep_s
     log(f"batch: {n_inst} joins, {rows / 1e9:.2f} G rows: separate {sep_s * 1e3:.2f} ms, "
-        f"eager {out['batch_eager_ms']:.2f} ms, graph {out['batch_graph_ms']:.2f} ms")
+        f"eager {out['batch_eager_ms']:.2f} ms, graph {out['batch_graph_ms']:.2f} ms, "
+        f"a launch per kind {out['batch_eager_per_kind_ms']:.2f} ms; filters {kinds}")
     del cols
     return out
 
@@ -123,11 +124,11 @@ def bench_single(ctx, reps=5):
     import torch
     from lipjoin import workloads as W
 
-    def timed(dfk, dm, nf, desc, method):
+    def timed(dfk, dm, nf, desc, method, sum_only=False):
         r = None
         ts = []
         for i in range(reps + 1):
-            r = ctx.hash_join(dfk, dm, nf, desc, method=method)
+            r = ctx.hash_join(dfk, dm, nf, desc, method=method, sum_only=sum_only)
             if i:
                 ts.append(r["time_probe"])
         st = r["stats"]
@@ -167,10 +168,17 @@ def bench_single(ctx, reps=5):
         e["lip_tuned"]["bloom_popcount"] = dim.info()["bloom_bits_set"]
         e["lip_tuned"]["bloom_words_xxh64"] = words_hash(words)
         dim.release()
-        assert e["lip_tuned"]["sum"] == e["naive"]["sum"] == e["lip_best"]["sum"]
+        # SUM(measure) only (the configuration's output; no product): the
+        # filter tuned for a semi-join, whose exact modulo filter decides the
+        # table probe
+        bs, _ = W.tuned_bloom(ctx, sp, semi_join=True)
+        e["lip_semi"] = dict(timed(dfk, dm, nf, L_dim(dk, nd, da, pred, bs), "lip", sum_only=True), bloom=str(bs))
+        assert e["lip_tuned"]["sum"] == e["naive"]["sum"] == e["lip_best"]["sum"] == e["lip_semi"]["sum"]
+        assert e["lip_tuned"]["n_match"] == e["lip_semi"]["n_match"]
         rows.append(e)
         log(f"config2 s={sel}%: naive {e['naive']['probe_ms']:.3f} ms, LIP best {e['lip_best']['probe_ms']:.3f} "
-            f"({best[0]}), tuned {e['lip_tuned']['probe_ms']:.3f} ({bl})")
+            f"({best[0]}), tuned {e['lip_tuned']['probe_ms']:.3f} ({bl}), sum-only {e['lip_semi']['probe_ms']:.3f} "
+            f"({bs})")
     out["config2"] = rows
     return out
 

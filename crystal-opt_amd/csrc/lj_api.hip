@@ -48,6 +48,7 @@ struct lj_dim {
   int64_t n_selected = -1;
   int64_t n_oob = -1;
   int64_t n_past = -1;     // resident: filter bits set past the staged prefix (-1: not known)
+  int64_t n_outside = -1;  // resident: selected keys from keys_min + ht_len on (-1: not known)
   float time_build_ms = 0.f;
   // tables taken from the zero pool (cleared behind their last use) go back
   // there; tables cleared inside the query's own window (lj_hash_join, the
@@ -95,6 +96,7 @@ struct lj_ctx {
   // tuning (environment overrides for sweeps: LJ_PROBE_WG, LJ_PROBE_VEC, LJ_LDS_POLICY, LJ_GROUP_SLOTS)
   int probe_wg = 0;
   int probe_vec = 0;
+  int exact_filter = 1;    // let an exact filter decide the table probe (LJ_EXACT_FILTER=0: off)
   int lds_policy = -1;
   int group_slots = 0;           // LJ_GROUP_SLOTS: the LDS group table's size, 0 = from the group space
   int filter_prefix = 1;         // LJ_FILTER_PREFIX=0: stage whole modulo filters
@@ -424,6 +426,7 @@ extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* 
     return bail(fail(c, LJ_EHIP, "hipHostGetDevicePointer failed"));
   c->probe_wg = env_int("LJ_PROBE_WG", 0);
   c->probe_vec = env_int("LJ_PROBE_VEC", 0);
+  c->exact_filter = env_int("LJ_EXACT_FILTER", 1);
   c->lds_policy = env_int("LJ_LDS_POLICY", -1);
   c->group_slots = env_int("LJ_GROUP_SLOTS", 0);
   c->filter_prefix = env_int("LJ_FILTER_PREFIX", 1);
@@ -680,6 +683,7 @@ extern "C" lj_status lj_dim_build(lj_ctx* c, const lj_dim_desc* desc, lj_dim** o
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   sum_counters(hc, h);
   D->n_past = (int64_t)hc[kCounterPast];
+  D->n_outside = (int64_t)hc[kCounterOutside];
   HIP_TRY(c, hipEventElapsedTime(&D->time_build_ms, c->ev[0], c->ev[1]));
   D->n_selected = (int64_t)h[0];
   D->n_oob = (int64_t)h[1];
@@ -807,7 +811,7 @@ struct QueryOut {
 // Workgroups per CU of a probe configuration, cached per context (cache
 // false: a cached value is used, a new one is not stored).
 static lj_status occupancy_of(lj_ctx* c, const ProbeConfig& cfg, size_t lds_bytes, int* per_cu, bool cache = true) {
-  const auto key = std::make_pair(cfg.wg * 1000 + cfg.vec * 100 + cfg.k0, lds_bytes);
+  const auto key = std::make_pair(cfg.wg * 100000 + cfg.vec * 1000 + cfg.k0, lds_bytes);
   auto it = c->occupancy.find(key);
   if (it == c->occupancy.end()) {
     HIP_TRY(c, probe_occupancy(cfg, lds_bytes, per_cu));
@@ -912,6 +916,22 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
     P.past_count = nullptr;
     P.bloom_lds_words = (int32_t)D->bloom_words;
     P.gmult = q->group_mult[d];
+  }
+  // A one-dimension LIP join whose result needs no payload (one group, no
+  // product) with a modulo filter of at least ht_len bits over the table's
+  // keys_min: the filter decides the table probe of every key inside the
+  // table's domain (exact_key in the probe), provided every selected key lies
+  // in that domain -- known here for a resident dimension, checked by the
+  // probe on the build's counters for one built with the query.
+  if (lip && nd == 1 && !want_product && q->n_groups == 1) {
+    ProbeDim& P = a.dim[0];
+    const lj_dim* D = p.dims[0];
+    const bool mod = P.bh.kind == BLOOM_MOD || P.bh.kind == BLOOM_MOD_POW2;
+    const bool pht = D->desc.ht_kind == LJ_HT_PHT1 || D->desc.ht_kind == LJ_HT_PHT2;
+    const bool known_in = !q->resident[0] || (D->n_oob == 0 && D->n_outside == 0);
+    if (c->exact_filter && mod && pht && P.bloom && P.bh.keys_min == P.keys_min && P.bh.mod.d >= P.ht_len.d &&
+        known_in)
+      P.exact = 1;
   }
   // Whole filters, or their prefixes (filter_prefix_words): prefixes free
   // LDS for a larger group table when the group space is small enough for
@@ -1049,6 +1069,7 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   a.method = lip ? 1 : 0;
   a.adaptive = adaptive ? 1 : 0;
   a.want_product = want_product;
+  a.k0 = cfg.k0;
   for (int i = 0; i < LJ_MAX_DIMS; i++) {
     a.order[i] = i < nd ? q->order[i] : i;
     a.load_order[i] = i < nd ? (lip ? p.bloom_order[i] : q->order[i]) : i;
@@ -1367,8 +1388,9 @@ extern "C" lj_status lj_hash_join(lj_ctx* c, const lj_join_desc* j, lj_join_resu
   for (int i = 0; i < LJ_MAX_DIMS; i++) q.bloom_order[i] = -1;
   q.dims[0] = j->dim;
   q.n_groups = 1;
+  if (j->flags & ~LJ_JOIN_SUM_ONLY) return fail(c, LJ_EINVAL, "unknown flags 0x%x", (unsigned)j->flags);
   QueryOut qo;
-  lj_status s = run_query(c, &q, 1, qo, true);
+  lj_status s = run_query(c, &q, (j->flags & LJ_JOIN_SUM_ONLY) ? 0 : 1, qo, true);
   if (s != LJ_OK && s != LJ_EDOMAIN) return s;
   memset(out, 0, sizeof *out);
   out->n_selected = qo.dim_selected[0];
@@ -1596,6 +1618,23 @@ static lj_status batch_build(lj_ctx* c, const lj_star_desc* descs, int32_t n, co
     K.lds = std::max(K.lds, bq.p.lds_bytes);
     K.qs.push_back(i);
     bq.group = k;
+  }
+  // queries of several stage-0 kinds at the default shape: one launch of the
+  // mixed-kind probe (each workgroup runs its query's specialised body)
+  if (b->kinds.size() > 1 && b->opts.launch_per_kind == 0) {
+    bool one_shape = true;
+    for (const BatchKind& K : b->kinds) one_shape = one_shape && K.cfg.wg == 1024 && K.cfg.vec == 1;
+    if (one_shape) {
+      BatchKind M{};
+      M.cfg = ProbeConfig{1024, 1, kS0Mixed};
+      for (int i = 0; i < n; i++) {
+        M.lds = std::max(M.lds, b->q[(size_t)i].p.lds_bytes);
+        M.qs.push_back(i);
+        b->q[(size_t)i].group = 0;
+      }
+      b->kinds.clear();
+      b->kinds.push_back(M);
+    }
   }
   // workgroups: each query gets as many as one query alone would (the
   // resident slots of the kind, fewer if it has fewer tiles), in query order.
@@ -1858,8 +1897,10 @@ static double fpr_of(int hash, int layout, int k, double n, double m, double dim
 static double lines_of(double p) { return 1.0 - std::pow(1.0 - p, 32.0); }
 
 extern "C" lj_status lj_tune_bloom(lj_ctx* c, int64_t n_selected, int64_t n_dim_rows, double fact_pass,
-                                   lj_bloom_choice* out) {
+                                   int32_t flags, lj_bloom_choice* out) {
   if (!c || !out) return fail(c, LJ_EINVAL, "NULL argument");
+  if (flags & ~LJ_TUNE_SEMI_JOIN) return fail(c, LJ_EINVAL, "unknown flags 0x%x", (unsigned)flags);
+  const bool semi = (flags & LJ_TUNE_SEMI_JOIN) != 0 && c->exact_filter;
   if (n_selected < 0 || n_dim_rows < 1 || n_selected > n_dim_rows || !(fact_pass >= 0.0 && fact_pass <= 1.0))
     return fail(c, LJ_EINVAL, "n_selected %lld, n_dim_rows %lld, fact_pass %g", (long long)n_selected,
                 (long long)n_dim_rows, fact_pass);
@@ -1880,7 +1921,9 @@ extern "C" lj_status lj_tune_bloom(lj_ctx* c, int64_t n_selected, int64_t n_dim_
       double scan = unit;
       if (cd.layout == LJ_BLOOM_STANDARD && cd.k > 1) scan += (cd.k - 1) * c->tune_probe_extra * unit;
       const double pass = fact_pass + (1.0 - fact_pass) * fpr;
-      const double cost = scan + lines_of(pass) * c->tune_line + pass * c->tune_gather;
+      // a semi-join's exact modulo filter decides the table probe: no gathers
+      const bool exact = semi && cd.hash == LJ_BLOOM_MOD && m >= (double)n_dim_rows;
+      const double cost = scan + lines_of(pass) * c->tune_line + (exact ? 0.0 : pass * c->tune_gather);
       if (cost < best.cost * (1.0 - 1e-9)) {
         best.bloom_hash = cd.hash;
         best.bloom_layout = cd.layout;
@@ -1974,12 +2017,18 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
     return st;
   };
   float t_lds = 0, t_l2 = 0, t_big = 0, t_p1 = 0, t_p40 = 0;
+  // the survivor costs are measured with every passing row reading the table
+  // (these one-group joins would otherwise let the exact filter decide)
+  const int exact_saved = c->exact_filter;
+  c->exact_filter = 0;
   if ((s = run(LJ_BLOOM_MULT, 1 << 20, 0, &t_lds)) != LJ_OK || (s = run(LJ_BLOOM_MOD, nd, 0, &t_l2)) != LJ_OK ||
       (s = run(LJ_BLOOM_MOD, 1 << 25, 0, &t_big)) != LJ_OK || (s = run(LJ_BLOOM_MOD, nd, 100, &t_p1)) != LJ_OK ||
       (s = run(LJ_BLOOM_MOD, nd, 4000, &t_p40)) != LJ_OK) {
+    c->exact_filter = exact_saved;
     release();
     return s;
   }
+  c->exact_filter = exact_saved;
   release();
   if (!(t_lds > 0 && t_l2 > 0 && t_big > 0)) return fail(c, LJ_EHIP, "calibration timings are not positive");
   c->tune_scan_gl = std::max(1.0, (double)t_l2 / t_lds);
@@ -2218,6 +2267,18 @@ static lj_status serve_launch(lj_server* S, ServeSlot& sl, std::vector<std::pair
       }
       kinds[k].push_back(i);
       klds[k] = std::max(klds[k], sl.reqs[i].p.lds_bytes);
+    }
+    // several stage-0 kinds at the default shape: one launch of the mixed-kind probe
+    bool one_shape = kinds.size() > 1;
+    for (const ProbeConfig& cf : kcfg) one_shape = one_shape && cf.wg == 1024 && cf.vec == 1;
+    if (one_shape) {
+      size_t lds = 0;
+      for (size_t x : klds) lds = std::max(lds, x);
+      std::vector<size_t> all;
+      for (size_t i = 0; i < sl.reqs.size(); i++) all.push_back(i);
+      kinds.assign(1, all);
+      kcfg.assign(1, ProbeConfig{1024, 1, kS0Mixed});
+      klds.assign(1, lds);
     }
     ProbeArgs* hq = reinterpret_cast<ProbeArgs*>(sl.args_host);
     uint32_t* hw = reinterpret_cast<uint32_t*>(sl.args_host + sl.wgb_off);

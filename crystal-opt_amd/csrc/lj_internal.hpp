@@ -66,8 +66,9 @@ struct BuildDim {
 // to pair w % kCounterLanes) so that the build's per-workgroup atomics do not
 // all serialize on one address; readers sum the pairs.
 constexpr int kCounterLanes = 16;
-constexpr int kCounterWords = 2 * kCounterLanes + 2;  // + popcount scratch, + kCounterPast
+constexpr int kCounterWords = 2 * kCounterLanes + 3;  // + popcount scratch, kCounterPast, kCounterOutside
 constexpr int kCounterPast = 2 * kCounterLanes + 1;   // selected keys whose filter bit is past the staged prefix
+constexpr int kCounterOutside = 2 * kCounterLanes + 2;  // selected PHT keys at or past keys_min + ht_len
 
 struct ProbeDim {
   const int32_t* fk;      // fact FK column of this dimension
@@ -84,6 +85,11 @@ struct ProbeDim {
   int64_t gmult;
   const unsigned long long* past_count;  // filter bits set past the staged prefix (device
                                          // counter of a dimension built with the query), or null
+  int32_t exact;          // LIP, one dimension, one group, no product: the filter is a modulo
+                          // filter of at least ht_len bits over the table's keys_min, so a
+                          // passing key in [keys_min, keys_min + ht_len), other than 0, hits
+                          // the table (see exact_key); its payload is not needed
+  int32_t reserved0;
 };
 
 struct ProbeArgs {
@@ -127,6 +133,8 @@ struct ProbeArgs {
   int64_t g_stride;
   int32_t g_reps;
   int32_t check_past;      // some dimension passes past_count: the probe reads it at its start
+  int32_t k0;              // the query's stage-0 kind (S0Kind; read by probe_mixed_kernel)
+  int32_t reserved_k0;
   // if set (a single query): the header and the first host_groups groups are
   // also written to this pinned host memory, so no copy follows the probe
   long long* host_hdr;
@@ -187,6 +195,9 @@ enum S0Kind : int {
   S0_GENERIC = 8,    // runtime dispatch (adaptive order over mixed filter kinds)
   S0_NKINDS = 9
 };
+// ProbeConfig.k0 of a launch whose queries have different stage-0 kinds
+// (probe_mixed_kernel: each query's ProbeArgs.k0 picks its body)
+constexpr int kS0Mixed = 100;
 
 struct ProbeConfig {
   int wg;      // threads per workgroup (256, 512, 1024)
@@ -200,6 +211,7 @@ struct ProbeEntry {
   const void* fn_multi;  // probe_multi_kernel (a batch of queries)
 };
 int probe_entries_pht(const ProbeEntry** out);
+int probe_entries_mixed(const ProbeEntry** out);
 int probe_entries_modp2(const ProbeEntry** out);
 int probe_entries_mod(const ProbeEntry** out);
 int probe_entries_mult(const ProbeEntry** out);

@@ -85,12 +85,12 @@ __device__ __forceinline__ bool tb_insert(int32_t* ht, uint32_t mask, int32_t ke
 
 // Workgroup `blk` of dimension B: rows [blk * 256 * rpt, (blk + 1) * 256 * rpt).
 __device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
-  __shared__ unsigned int cnt[3];
-  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+  __shared__ unsigned int cnt[4];
+  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
   __syncthreads();
   const int rpt = build_rows_per_thread(B.n);
   const int64_t r0 = (blk * kBuildThreads + threadIdx.x) * rpt;
-  uint32_t nsel = 0, noob = 0, npast = 0;
+  uint32_t nsel = 0, noob = 0, npast = 0, nout = 0;
 #pragma unroll
   for (int j = 0; j < kBuildRows; j++) {
     const int64_t r = r0 + j;
@@ -112,11 +112,16 @@ __device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
           const int64_t s = pht_slot(key, B.ht_len, B.keys_min);
           if (s < 0) {
             noob++;
-          } else if (B.ht_kind == HT_PHT1) {
-            atomicCAS(&B.ht[s], 0, key);
           } else {
-            atomicCAS(&B.ht[2 * s], 0, key);
-            B.ht[2 * s + 1] = B.payload[r];
+            // a key from keys_min + ht_len on wraps into the table (the
+            // reference's HASH); counted, inserted like the others
+            if ((int64_t)key - (int64_t)B.keys_min >= (int64_t)B.ht_len.d) nout++;
+            if (B.ht_kind == HT_PHT1) {
+              atomicCAS(&B.ht[s], 0, key);
+            } else {
+              atomicCAS(&B.ht[2 * s], 0, key);
+              B.ht[2 * s + 1] = B.payload[r];
+            }
           }
         }
       }
@@ -125,10 +130,12 @@ __device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
   nsel = WaveSum(nsel);
   noob = WaveSum(noob);
   npast = WaveSum(npast);
+  nout = WaveSum(nout);
   if (__lane_id() == 0) {
     if (nsel) atomicAdd(&cnt[0], nsel);
     if (noob) atomicAdd(&cnt[1], noob);
     if (npast) atomicAdd(&cnt[2], npast);
+    if (nout) atomicAdd(&cnt[3], nout);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -136,6 +143,7 @@ __device__ __forceinline__ void build_block(const BuildDim& B, int64_t blk) {
     if (cnt[0]) atomicAdd(&c[0], (unsigned long long)cnt[0]);
     if (cnt[1]) atomicAdd(&c[1], (unsigned long long)cnt[1]);
     if (cnt[2]) atomicAdd(&B.counters[kCounterPast], (unsigned long long)cnt[2]);
+    if (cnt[3]) atomicAdd(&B.counters[kCounterOutside], (unsigned long long)cnt[3]);
   }
 }
 

@@ -19,7 +19,7 @@ size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words, int group_slo
 
 static const ProbeEntry* find_entry(const ProbeConfig& c) {
   int (*tables[])(const ProbeEntry**) = {probe_entries_pht, probe_entries_modp2, probe_entries_mod,
-                                         probe_entries_mult, probe_entries_generic};
+                                         probe_entries_mult, probe_entries_generic, probe_entries_mixed};
   for (auto t : tables) {
     const ProbeEntry* e = nullptr;
     const int n = t(&e);

@@ -199,17 +199,28 @@ __device__ __forceinline__ bool pht_probe(const ProbeDim& D, int32_t key, int32_
   return ht_resolve(D.ht_kind, D.ht, D.ht_len, key, s, ht_load(D.ht_kind, D.ht, s), pay);
 }
 
+// A key whose table probe an exact filter decides (D.exact): with x =
+// key - keys_min in [0, ht_len), the filter's bit x and the table's slot x
+// are the same position, and every selected dimension key is inside that
+// domain (the probe checks the build's out-of-domain count), so the bit is
+// set iff the key was inserted.  Key 0 is left to the table: its pair may be
+// all zero, which the reference reads as a miss (the zero sentinel).
+__device__ __forceinline__ bool exact_key(const ProbeDim& D, int32_t key) {
+  return (uint32_t)key - (uint32_t)D.keys_min < D.ht_len.d && key != 0;
+}
+
 // A filter probe outside stage 0 (later stages, the adaptive sample).  When
 // every filter of the plan has stage 0's kind and placement (all_k0), the
 // probe is that kind's short test; otherwise the generic test.  Besides the
 // work, this keeps the executed code small: the generic test inlined at each
 // call site is thousands of instructions, and walking them misses the
 // instruction cache.  SPEC: 0 tests all_k0 here, 1 assumes it (the caller
-// tested it once for a whole loop), 2 assumes it is not set.
+// tested it once for a whole loop), 2 assumes it is not set, 3 is the
+// one-dimension semi-join body (one filter: all_k0 holds).
 template <int K0, int SPEC = 0>
 __device__ __forceinline__ bool probe_filter(const ProbeArgs& a, const ProbeDim& D, const uint32_t* lds,
                                              int32_t key) {
-  const bool spec = SPEC == 1 || (SPEC == 0 && a.all_k0);
+  const bool spec = SPEC == 1 || SPEC == 3 || (SPEC == 0 && a.all_k0);
   if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_GL) {
     if (spec) {
       const uint32_t x = (uint32_t)key - (uint32_t)D.bh.keys_min;
@@ -237,7 +248,7 @@ __device__ __forceinline__ bool probe_filter(const ProbeArgs& a, const ProbeDim&
 template <int K0, int SPEC>
 __device__ __forceinline__ uint32_t probe_filter4(const ProbeArgs& a, const ProbeDim& D, const uint32_t* lds,
                                                   const v4i32& kv, uint32_t mask4) {
-  const bool spec = SPEC == 1 || (SPEC == 0 && a.all_k0);
+  const bool spec = SPEC == 1 || SPEC == 3 || (SPEC == 0 && a.all_k0);
   if constexpr (K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_GL) {
     if (spec) {
       const int32_t kk[4] = {kv.x, kv.y, kv.z, kv.w};
@@ -564,6 +575,9 @@ __device__ __forceinline__ int batch_lines(bool alive, int32_t row, int32_t& las
 // ---- per-wave state (named scalars: accessed by compile-time stage numbers) ---
 struct WaveCtx {
   int nd;
+  bool exact;               // one dimension whose exact filter replaces the table probe
+  long long acc_s;          // exact: the lane's SUM(measure) and row count, added in place
+  uint32_t acc_c;
   int ld0, ld1, ld2, ld3;   // column load order (stage -> dim)
   int32_t* q_row;
   int32_t* q_key;
@@ -597,7 +611,7 @@ struct WaveCtx {
 // stage 1's when bit 31 of row says the row passed stage 1 already (the
 // survivors of a dense segment).  With one dimension, bit 31 says that x1 is
 // the row's measure (read from the streamed measure column).
-template <int K0>
+template <int K0, int SPEC>
 __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int cnt) {
   constexpr bool LIP = K0 != S0_PHT;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -625,8 +639,12 @@ __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int 
     m = r < 0 ? x1 : gl(a.measure)[row];
     if (LIP) {
       const ProbeDim& D = a.dim[0];
-      const int64_t sl = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, x0);
-      if (sl >= 0) ht = ht_load(D.ht_kind, D.ht, sl);
+      if (SPEC == 3 && w.exact && exact_key(D, x0)) {
+        ht = 1ull;  // a hit (payload 0) that the filter decided
+      } else {
+        const int64_t sl = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, x0);
+        if (sl >= 0) ht = ht_load(D.ht_kind, D.ht, sl);
+      }
     }
   }
   w.p_ht = ht;
@@ -764,7 +782,7 @@ __device__ __forceinline__ void rotate_batch(const ProbeArgs& a, const uint32_t*
                                              WaveStats& st) {
   complete_batch<K0, SPEC>(a, lds, gt, w, st);
   const int take = w.qcount < kQueue ? w.qcount : kQueue;
-  issue_batch<K0>(a, w, take);
+  issue_batch<K0, SPEC>(a, w, take);
   const int rest = w.qcount - take;
   if (rest > 0) {
     // reads from [kQueue, kQueue + rest), writes to [0, rest): no overlap, and
@@ -939,6 +957,30 @@ __device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* pa
   }
 }
 
+// One dimension with an exact filter (w.exact), its measure streamed with the
+// keys: a segment whose stage-0 survivors all have keys the filter decides is
+// finished in place -- every survivor is a hit; the lane adds SUM(measure)
+// and the row count to its running totals (one group), flushed once at the
+// end.  Returns false (nothing done) for a segment with a key the filter
+// cannot decide (key 0, or outside the table's domain): its survivors take
+// the queue, which reads the table for those keys.
+__device__ __forceinline__ bool exact_in_place(const ProbeArgs& a, WaveCtx& w, WaveStats& st, const v4i32& kv,
+                                               const v4i32& mv, uint32_t m) {
+  const ProbeDim& D = a.dim[0];
+  bool undecided = false;
+#pragma unroll
+  for (int j = 0; j < 4; j++) undecided = undecided || (((m >> j) & 1u) && !exact_key(D, vget(kv, j)));
+  if (__ballot(undecided)) return false;
+  st.meas_rows += popc_ballot4(m);
+  st.meas_lines += lines_of_ballot(__ballot(m != 0));
+  long long s = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) s += ((m >> j) & 1u) ? (long long)vget(mv, j) : 0ll;
+  w.acc_s += s;
+  w.acc_c += (uint32_t)__popc(m);
+  return true;
+}
+
 struct TileCtx {
   uint32_t n;        // fact rows
   uint32_t start;    // first row of the tile space (after the adaptive sample)
@@ -1003,6 +1045,10 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
     for (int j = 0; j < 4; j++) {
       qx0[j] = LIP ? vget(kv, j) : pv[j];
       qx1[j] = 0;
+    }
+    if (SPEC == 3 && have_eager && w.exact && exact_in_place(a, w, st, kv, ev, m)) {
+      st.v[ST_BLOOM_PASS] += cnt;  // every survivor passed the only filter
+      continue;
     }
     if (cnt > kDenseMin || have_eager) {
       // with two or more dimensions stage 1 runs here on the streamed column;
@@ -1319,6 +1365,17 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   w.ld1 = a.load_order[1];
   w.ld2 = a.load_order[2];
   w.ld3 = a.load_order[3];
+  w.acc_s = 0;
+  w.acc_c = 0;
+  w.exact = SPEC == 3 && a.n_dims == 1 && a.dim[0].exact != 0;
+  if (w.exact && a.dim_counters[0]) {
+    // a dimension built with the query: a selected key outside the table's
+    // domain (below keys_min, or aliased from keys_min + ht_len on) would
+    // break the bit-slot match
+    unsigned long long o = a.dim_counters[0][kCounterOutside];
+    for (int l = 0; l < kCounterLanes; l++) o += a.dim_counters[0][2 * l + 1];
+    w.exact = o == 0;
+  }
 
   const uint32_t total_waves = nblk * NW;
   const uint32_t gw = bid * NW + wave;
@@ -1405,6 +1462,18 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   for (int it = 0; it < 2; it++) rotate_batch<K0, SPEC>(a, lds_filter, gt, w, st);
 
   // ---- flush: statistics and groups ----------------------------------------------
+  if (SPEC == 3 && w.exact) {
+    const long long es = WaveSum(w.acc_s);
+    const unsigned long long ec = WaveSum((unsigned long long)w.acc_c);
+    if (lane == 0 && ec != 0) {
+      atomicAdd(&gt.sum[0], (unsigned long long)es);
+      atomicAdd(&gt.cnt[0], ec);
+      if (gt.key[0] == kEmptyGroup) {
+        gt.key[0] = 0;
+        for (int d = 0; d < kMaxDims; d++) gt.pay[d] = 0;
+      }
+    }
+  }
   {
     int o = oob;
 #pragma unroll
@@ -1472,6 +1541,15 @@ __device__ __forceinline__ void probe_dispatch(const ProbeArgs& a, uint32_t bid,
           a.dim[d].bloom_lds_words < a.dim[d].bloom_words)
         past = past || *a.dim[d].past_count != 0ull;
   }
+  constexpr bool is_mod = K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_GL;
+  if constexpr (is_mod) {
+    // a one-dimension semi-join through an exact filter: a body of its own
+    // (its in-place sums cost registers the other bodies do not pay for)
+    if (a.n_dims == 1 && a.dim[0].exact) {
+      probe_body<WG, VEC, K0, 3>(a, bid, nblk);
+      return;
+    }
+  }
   if constexpr (has_spec) {
     if (a.all_k0 && !past) probe_body<WG, VEC, K0, 1>(a, bid, nblk);
     else probe_body<WG, VEC, K0, 2>(a, bid, nblk);
@@ -1497,22 +1575,30 @@ __global__ __launch_bounds__(WG) void probe_kernel(ProbeArgs args) {
 // [wg_begin[q], wg_begin[q+1]) (wg_begin[n_q] == gridDim.x); each workgroup
 // finds its query by binary search and runs the same probe on the query's
 // arguments, read from device memory.
-template <int WG, int VEC, int K0>
-__global__ __launch_bounds__(WG) void probe_multi_kernel(const ProbeArgs* __restrict__ qs,
-                                                         const uint32_t* __restrict__ wg_begin, int32_t n_q) {
-  // __restrict__: the arguments do not alias what the probe writes, so their
-  // fields load once instead of after every store
+__device__ __forceinline__ int query_of_block(const uint32_t* __restrict__ wg_begin, int32_t n_q) {
   int lo = 0, hi = n_q - 1;
   while (lo < hi) {  // last q with wg_begin[q] <= blockIdx.x (uniform)
     const int mid = (lo + hi + 1) >> 1;
     if (wg_begin[mid] <= blockIdx.x) lo = mid;
     else hi = mid - 1;
   }
-  const uint32_t b0 = wg_begin[lo];
-  probe_dispatch<WG, VEC, K0>(qs[lo], blockIdx.x - b0, wg_begin[lo + 1] - b0);
+  return lo;
 }
 
-// Host-side table of instantiated kernels, filled by the instantiation units.
+template <int WG, int VEC, int K0>
+__global__ __launch_bounds__(WG) void probe_multi_kernel(const ProbeArgs* __restrict__ qs,
+                                                         const uint32_t* __restrict__ wg_begin, int32_t n_q) {
+  // __restrict__: the arguments do not alias what the probe writes, so their
+  // fields load once instead of after every store
+  const int q = query_of_block(wg_begin, n_q);
+  const uint32_t b0 = wg_begin[q];
+  probe_dispatch<WG, VEC, K0>(qs[q], blockIdx.x - b0, wg_begin[q + 1] - b0);
+}
+
+// Host-side table of instantiated kernels, filled by the instantiation units
+// (a build may name other shapes with -DLJ_PROBE_SHAPES(X,K)=...).
+#ifndef LJ_PROBE_SHAPES
 #define LJ_PROBE_SHAPES(X, K) X(256, 1, K) X(512, 1, K) X(1024, 1, K)
+#endif
 
 }  // namespace lj

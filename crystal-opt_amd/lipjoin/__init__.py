@@ -16,8 +16,8 @@ from . import _abi
 from ._abi import (  # noqa: F401  (re-exported constants)
     BLOOM_BLOCKED, BLOOM_MOD, BLOOM_MULT, BLOOM_NONE, BLOOM_STANDARD, CMP_EQ, CMP_GT, CMP_GTE,
     CMP_LT, CMP_LTE, COMB_AND, COMB_OR, COMB_SET, GEN_DATEKEY, GEN_DATEKEY_SEQ, GEN_SEQUENCE,
-    GEN_UNIFORM, GEN_YEAR_OF_DAY, HT_PHT1, HT_PHT2, HT_TEXTBOOK, METHOD_LIP, METHOD_LIP_ADAPTIVE,
-    METHOD_NAIVE, MAX_DIMS, LibraryMissing,
+    GEN_UNIFORM, GEN_YEAR_OF_DAY, HT_PHT1, HT_PHT2, HT_TEXTBOOK, JOIN_SUM_ONLY, METHOD_LIP,
+    METHOD_LIP_ADAPTIVE, METHOD_NAIVE, MAX_DIMS, TUNE_SEMI_JOIN, LibraryMissing,
 )
 
 METHODS = {"naive": METHOD_NAIVE, "lip": METHOD_LIP, "lip-adaptive": METHOD_LIP_ADAPTIVE}
@@ -182,11 +182,14 @@ class Context:
         return Dim(self, h)
 
     # -- single join (hashJoin) -------------------------------------------------------
-    def hash_join(self, fact_fk, fact_measure, n_fact, dim, method="lip", allow_domain=False):
+    def hash_join(self, fact_fk, fact_measure, n_fact, dim, method="lip", allow_domain=False, sum_only=False):
+        """lj_hash_join; sum_only: SUM(measure) and the counts only
+        (LJ_JOIN_SUM_ONLY: sum_product is 0)."""
         j = _abi.JoinDesc()
         j.fact_fk, j.fact_measure, j.n_fact = _ptr(fact_fk), _ptr(fact_measure), int(n_fact)
         j.dim = dim
         j.method = METHODS[method] if isinstance(method, str) else method
+        j.flags = JOIN_SUM_ONLY if sum_only else 0
         r = _abi.JoinResult()
         st = self._check(lib().lj_hash_join(self.h, C.byref(j), C.byref(r)),
                          allow=(_abi.EDOMAIN,) if allow_domain else ())
@@ -238,17 +241,21 @@ class Context:
         }
 
     # -- batches (many queries, one build launch + one probe launch per kernel kind) --------
-    def batch(self, descs, use_graph=False, groups_cap=64, probe_wg=0):
-        return Batch(self, descs, use_graph, groups_cap, probe_wg)
+    def batch(self, descs, use_graph=False, groups_cap=64, probe_wg=0, launch_per_kind=False):
+        return Batch(self, descs, use_graph, groups_cap, probe_wg, launch_per_kind)
 
     # -- serving (request loop over resident dimensions) ----------------------------------
     def serve(self, hbm_budget=0, max_batch=64, max_inflight=2, groups_cap=64, streams=1):
         return Server(self, hbm_budget, max_batch, max_inflight, groups_cap, streams)
 
     # -- Bloom autotuner ------------------------------------------------------------------
-    def tune_bloom(self, n_selected, n_dim_rows, fact_pass):
+    def tune_bloom(self, n_selected, n_dim_rows, fact_pass, semi_join=False):
+        """lj_tune_bloom; semi_join: the join needs no payload (one group, no
+        product), so an exact modulo filter costs no hash-table gathers."""
         ch = _abi.BloomChoice()
-        self._check(lib().lj_tune_bloom(self.h, int(n_selected), int(n_dim_rows), float(fact_pass), C.byref(ch)))
+        flags = TUNE_SEMI_JOIN if semi_join else 0
+        self._check(lib().lj_tune_bloom(self.h, int(n_selected), int(n_dim_rows), float(fact_pass), flags,
+                                        C.byref(ch)))
         return {k: getattr(ch, k) for k, _ in _abi.BloomChoice._fields_}
 
     def tune_calibrate(self):
@@ -272,17 +279,18 @@ def _star_result_dict(r, arr, n_dims, cap):
 
 
 class Batch:
-    """lj_batch: the queries run as one build launch and one probe launch per
-    stage-0 kernel kind; with use_graph the launches are recorded once as a HIP
-    graph and replayed."""
+    """lj_batch: the queries run as one build launch and one probe launch
+    (launch_per_kind: one probe launch per stage-0 kernel kind); with use_graph
+    the launches are recorded once as a HIP graph and replayed."""
 
-    def __init__(self, ctx, descs, use_graph=False, groups_cap=64, probe_wg=0):
+    def __init__(self, ctx, descs, use_graph=False, groups_cap=64, probe_wg=0, launch_per_kind=False):
         self.ctx = ctx
         self.n = len(descs)
         self._descs = (_abi.StarDesc * self.n)(*descs)
         self._n_dims = [d.n_dims for d in descs]
         o = _abi.BatchOpts()
         o.use_graph, o.probe_wg, o.groups_cap = int(bool(use_graph)), int(probe_wg), int(groups_cap)
+        o.launch_per_kind = int(bool(launch_per_kind))
         self.cap = int(groups_cap)
         h = C.c_void_p()
         ctx._check(lib().lj_batch_create(ctx.h, self._descs, self.n, C.byref(o), C.byref(h)))

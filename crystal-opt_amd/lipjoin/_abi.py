@@ -29,6 +29,8 @@ BLOOM_STANDARD, BLOOM_BLOCKED = 0, 1
 METHOD_NAIVE, METHOD_LIP, METHOD_LIP_ADAPTIVE = 0, 1, 2
 # generators
 GEN_UNIFORM, GEN_DATEKEY, GEN_SEQUENCE, GEN_YEAR_OF_DAY, GEN_DATEKEY_SEQ = 0, 1, 2, 3, 4
+JOIN_SUM_ONLY = 1        # lj_join_desc.flags
+TUNE_SEMI_JOIN = 1       # lj_tune_bloom flags
 
 
 class PredStep(C.Structure):
@@ -109,7 +111,7 @@ class StarResult(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("use_graph", C.c_int32), ("probe_wg", C.c_int32), ("groups_cap", C.c_int64),
-                ("reserved", C.c_int64 * 2)]
+                ("launch_per_kind", C.c_int32), ("reserved0", C.c_int32), ("reserved", C.c_int64 * 1)]
 
 
 class BloomChoice(C.Structure):
@@ -153,7 +155,7 @@ SIGNATURES = {
                                   C.POINTER(C.c_void_p)]),
     "lj_batch_run": (C.c_int, [C.c_void_p, C.POINTER(StarResult)]),
     "lj_batch_destroy": (None, [C.c_void_p]),
-    "lj_tune_bloom": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.POINTER(BloomChoice)]),
+    "lj_tune_bloom": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_int32, C.POINTER(BloomChoice)]),
     "lj_tune_calibrate": (C.c_int, [C.c_void_p]),
     "lj_serve_start": (C.c_int, [C.c_void_p, C.POINTER(ServeCfg), C.POINTER(C.c_void_p)]),
     "lj_serve_submit": (C.c_int, [C.c_void_p, C.POINTER(StarDesc), C.POINTER(C.c_uint64)]),

@@ -8,7 +8,8 @@ Workloads (synthetic, seeded, generated on the device):
   ssb:<query>:<plan>    another SSB query (q21 ... q43) at 120,000,000 rows over SF-20 tables
   config1               the plain join: 2^20-row dimension, 2^24 fact rows, naive
   config2:<sel>:<bloom> the Bloom join at 2^28 rows, selectivity <sel> %, bloom "tuned",
-                        "none" (naive) or "mult-<lg>-<k>" / "mod-<lg>"
+                        "semi" (SUM(measure) only, filter tuned for it), "none" (naive)
+                        or "mult-<lg>-<k>" / "mod-<lg>"
   batch                 the 64-instance batch (one graph replay per repetition)
 --calib-copy-mb M first runs a plain device copy of M MiB (a reference for the HBM
 counters: it reads and writes M MiB).  Each workload line printed to stderr names
@@ -71,8 +72,8 @@ def _config2(ctx, sel, bloom_s, reps):
     seed, nd, nf = W.seed_of(2), 1 << 22, 1 << 28
     dk, da, dfk, dm = W.single_join_device(ctx, seed, nd, nf, 9999)
     thr = 100 * int(sel)
-    if bloom_s == "tuned":
-        bloom, _ = W.tuned_bloom(ctx, {"n_dim": nd, "threshold": thr})
+    if bloom_s in ("tuned", "semi"):
+        bloom, _ = W.tuned_bloom(ctx, {"n_dim": nd, "threshold": thr}, semi_join=bloom_s == "semi")
     elif bloom_s == "none":
         bloom = None
     else:
@@ -80,8 +81,9 @@ def _config2(ctx, sel, bloom_s, reps):
         bloom = ("mod", 1 << int(parts[1])) if parts[0] == "mod" else ("mult", 1 << int(parts[1]), int(parts[2]), 0)
     desc = dim_desc(dk, nd, nd, attr=da, predicate=W.single_join_pred(thr), bloom=bloom)
     for _ in range(reps):
-        r = ctx.hash_join(dfk, dm, nf, desc, method="naive" if bloom is None else "lip")
-    return f"config2 s={sel}% {bloom}: probe {r['time_probe']:.3f} ms, matches {r['n_match']}"
+        r = ctx.hash_join(dfk, dm, nf, desc, method="naive" if bloom is None else "lip",
+                          sum_only=bloom_s == "semi")
+    return f"config2 s={sel}% {bloom_s} {bloom}: probe {r['time_probe']:.3f} ms, matches {r['n_match']}"
 
 
 def _batch(ctx, reps):
@@ -91,7 +93,7 @@ def _batch(ctx, reps):
     cols, descs = [], []
     for sp in specs:
         c = W.batch_instance_device(ctx, sp)
-        bloom, _ = W.tuned_bloom(ctx, sp)
+        bloom, _ = W.tuned_bloom(ctx, sp, semi_join=True)
         cols.append(c)
         descs.append(W.batch_desc(c, sp, method="lip", bloom=bloom))
     torch.cuda.synchronize()

@@ -228,10 +228,11 @@ def batch_desc(cols, spec, method="lip", bloom=None):
     return star_desc([cols["fk"]], cols["meas"], spec["n_fact"], [d], (0,), method=method)
 
 
-def tuned_bloom(ctx, spec):
-    """The autotuner's Bloom filter for an instance, as a dim_desc bloom tuple."""
+def tuned_bloom(ctx, spec, semi_join=False):
+    """The autotuner's Bloom filter for an instance, as a dim_desc bloom tuple.
+    semi_join: the join needs no payload (one group, no product)."""
     n_sel = int(round(spec["n_dim"] * spec["threshold"] / 10000.0))
-    ch = ctx.tune_bloom(n_sel, spec["n_dim"], spec["threshold"] / 10000.0)
+    ch = ctx.tune_bloom(n_sel, spec["n_dim"], spec["threshold"] / 10000.0, semi_join=semi_join)
     if ch["bloom_hash"] == BLOOM_MOD:
         return ("mod", ch["bloom_bits"]), ch
     return ("mult", ch["bloom_bits"], ch["bloom_k"], ch["bloom_layout"]), ch
@@ -251,7 +252,7 @@ def serve_templates(ctx, seed, n_dim=1 << 22, n_templates=8, torch_device="cuda"
     for t in range(n_templates):
         thr = int(round(100 * rng.uniform(0.5, 20.0)))
         sp = dict(spec, threshold=thr)
-        bloom, _ = tuned_bloom(ctx, sp)
+        bloom, _ = tuned_bloom(ctx, sp, semi_join=True)    # requests sum one group
         d = dim_desc(cols["keys"], n_dim, n_dim, attr=cols["attr"], predicate=single_join_pred(thr), bloom=bloom)
         out.append({"dim": ctx.build_dim(d), "threshold": thr, "bloom": bloom})
     return cols, out

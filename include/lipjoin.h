@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define LJ_ABI_VERSION 3
+#define LJ_ABI_VERSION 4
 #define LJ_MAX_DIMS 4
 #define LJ_MAX_PRED_STEPS 4
 
@@ -181,8 +181,15 @@ typedef struct {
   int64_t n_fact;
   lj_dim_desc dim;             /* built inside the call (hashJoin semantics) */
   int32_t method;              /* NAIVE: no Bloom; LIP: Bloom then PHT_2 */
-  int32_t flags;               /* 0 */
+  int32_t flags;               /* 0, or LJ_JOIN_SUM_ONLY */
 } lj_join_desc;
+
+/* lj_join_desc.flags: report SUM(measure) and the counts only; sum_product
+ * (the reference's vals * join_vals aggregate, join.cu:96) is then 0.  With no
+ * payload needed, a LIP join whose modulo filter has at least ht_len bits lets
+ * the filter decide the table probe of every in-domain key (a semi-join:
+ * identical counts and sums, no hash-table reads for those keys). */
+#define LJ_JOIN_SUM_ONLY 1
 
 typedef struct {
   int64_t n_selected;     /* dimension rows selected */
@@ -287,7 +294,11 @@ typedef struct {
   int32_t use_graph;       /* record the launches as a HIP graph on the first run */
   int32_t probe_wg;        /* workgroup size 256/512/1024; 0: the context's choice */
   int64_t groups_cap;      /* result groups kept per query on the device (>= 1; 0: 64) */
-  int64_t reserved[2];
+  int32_t launch_per_kind; /* 0: one probe launch for the whole batch (queries of different
+                              stage-0 kinds share it at the 1024-thread shape); 1: one probe
+                              launch per stage-0 kind */
+  int32_t reserved0;
+  int64_t reserved[1];
 } lj_batch_opts;
 
 /* descs[0..n): every dimension built inside the batch (dims[d]) or resident.
@@ -382,7 +393,11 @@ typedef struct {
 } lj_bloom_choice;
 
 lj_status lj_tune_bloom(lj_ctx* ctx, int64_t n_selected, int64_t n_dim_rows, double fact_pass,
-                        lj_bloom_choice* out);
+                        int32_t flags, lj_bloom_choice* out);
+/* lj_tune_bloom flags: the join needs no payload (one group, no product), so
+ * an exact modulo filter (bits >= n_dim_rows, dense keys) decides the table
+ * probe and its passing rows cost no hash-table gather. */
+#define LJ_TUNE_SEMI_JOIN 1
 /* Measures the model's constants on this device (a few probes of synthetic
  * data, about a second) and keeps them in the context. */
 lj_status lj_tune_calibrate(lj_ctx* ctx);

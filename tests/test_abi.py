@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match():
     assert _abi.check_layout() == []
-    assert _abi.load().lj_abi_version() == 3
+    assert _abi.load().lj_abi_version() == 4
 
 
 def test_status_strings():

@@ -58,11 +58,14 @@ def _static_stats(r):
     return (st["rows_in"], st["lines_in"], st["rows_bloom_pass"])
 
 
-def test_batch_equals_single_and_oracle(ctx, small_batch):
+@pytest.mark.parametrize("per_kind", [False, True], ids=["one-launch", "launch-per-kind"])
+def test_batch_equals_single_and_oracle(ctx, small_batch, per_kind):
+    """Eight instances of six stage-0 kernel kinds: one probe launch (the
+    mixed-kind kernel) or one per kind, each run three times."""
     specs, cols, descs = small_batch
-    _flush("batch eager vs single vs oracle")
+    _flush(f"batch eager vs single vs oracle (per_kind={per_kind})")
     singles = [ctx.star_join(d) for d in descs]
-    b = ctx.batch(descs)
+    b = ctx.batch(descs, launch_per_kind=per_kind)
     for it in range(3):
         res = b.run()
         for i, (r, s) in enumerate(zip(res, singles)):
@@ -166,7 +169,7 @@ def test_batch_config4_full_shape_vs_oracle(ctx):
     cols, descs, blooms = [], [], []
     for sp in specs:
         c = W.batch_instance_device(ctx, sp)
-        bloom, _ = W.tuned_bloom(ctx, sp)
+        bloom, _ = W.tuned_bloom(ctx, sp, semi_join=True)
         cols.append(c)
         blooms.append(bloom)
         descs.append(W.batch_desc(c, sp, method="lip", bloom=bloom))

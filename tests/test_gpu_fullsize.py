@@ -123,6 +123,15 @@ def test_config2_every_selectivity(ctx, config2, sel):
                         threads=THREADS)
     assert (r["n_selected"], r["n_bloom_pass"], r["n_match"], r["sum_measure"], r["sum_product"]) == \
         (ref["n_selected"], ref["n_bloom_pass"], ref["n_match"], ref["sum_measure"], ref["sum_product"])
+    # SUM(measure) only, the filter tuned for a semi-join (an exact modulo
+    # filter decides the table probe of in-domain keys)
+    bs, _ = W.tuned_bloom(ctx, {"n_dim": nd, "threshold": thr}, semi_join=True)
+    rs = ctx.hash_join(dfk, dm, nf, L.dim_desc(dk, nd, nd, attr=da, predicate=pred, bloom=bs), method="lip",
+                       sum_only=True)
+    refs = ref if bs == bloom else O.single_join(O.DimSpec(keys, attr, nd, pred=list(pred), bloom=_obloom(bs)), fk,
+                                                 meas, method=O.LIP, threads=THREADS)
+    assert (rs["n_bloom_pass"], rs["n_match"], rs["sum_measure"], rs["sum_product"]) == \
+        (refs["n_bloom_pass"], refs["n_match"], refs["sum_measure"], 0)
     if sel in (1, 50):
         rn = ctx.hash_join(dfk, dm, nf, L.dim_desc(dk, nd, nd, attr=da, predicate=pred), method="naive")
         assert (rn["n_match"], rn["sum_measure"], rn["sum_product"]) == \
