@@ -454,10 +454,8 @@ struct GroupLds {
   int32_t* pay;             // [slots * 4]
 };
 
-// pay.v[s] is the payload of dimension dm.v[s] (a stage's dimension, or the
-// identity map).
 __device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, long long s, unsigned long long c,
-                                                 unsigned long long p, const Quad& pay, const Quad& dm, int nd) {
+                                                 unsigned long long p, const Quad& pay, int nd) {
   if (a.g_reps > 1) {
     // this workgroup's copy; the first add of a group to any copy lists it once
     const int64_t gi = g + (int64_t)(blockIdx.x & (uint32_t)(a.g_reps - 1)) * a.g_stride;
@@ -467,7 +465,7 @@ __device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, 
     if (old == 0 && atomicExch(&a.gseen[g], 1u) == 0u) {
 #pragma unroll
       for (int t = 0; t < kMaxDims; t++)
-        if (t < nd) a.gpay[g * kMaxDims + dm.v[t]] = pay.v[t];
+        if (t < nd) a.gpay[g * kMaxDims + t] = pay.v[t];
       uint32_t idx = atomicAdd(a.n_touched, 1u);
       if (idx < a.touched_cap) a.touched[idx] = (uint32_t)g;
     }
@@ -479,7 +477,7 @@ __device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, 
   if (old == 0) {
 #pragma unroll
     for (int t = 0; t < kMaxDims; t++)
-      if (t < nd) a.gpay[g * kMaxDims + dm.v[t]] = pay.v[t];
+      if (t < nd) a.gpay[g * kMaxDims + t] = pay.v[t];
     uint32_t idx = atomicAdd(a.n_touched, 1u);
     if (idx < a.touched_cap) a.touched[idx] = (uint32_t)g;
   }
@@ -487,13 +485,10 @@ __device__ __forceinline__ void global_group_add(const ProbeArgs& a, int64_t g, 
 
 // Add one row per alive lane to the workgroup's group table.  Called by every
 // lane of the wave; returns the number of rows (wave-wide) whose group index
-// fell outside [0, n_groups).  pay.v[s] is the payload of stage s's dimension
-// dm.v[s] (wave-uniform), so no register is selected by a dimension index.
-__device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool alive, const Quad& pay,
-                                         const Quad& dm, int32_t m) {
+// fell outside [0, n_groups).  pay.v[d] is dimension d's payload.
+__device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool alive, const Quad& pay, int32_t m) {
   const int nd = a.n_dims;
-  // the product's payload is dimension 0's (single joins: stage 0)
-  const int32_t pay0 = dm.v[0] == 0 ? pay.v[0] : dm.v[1] == 0 ? pay.v[1] : dm.v[2] == 0 ? pay.v[2] : pay.v[3];
+  const int32_t pay0 = pay.v[0];  // the product's payload is dimension 0's
   if (a.n_groups == 1) {
     // one group (single join): wave-reduce, one LDS atomic per wave
     long long s = alive ? (long long)m : 0;
@@ -510,7 +505,7 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
       if (gt.key[0] == kEmptyGroup) {
         gt.key[0] = 0;
         for (int t = 0; t < kMaxDims; t++)
-          if (t < nd) gt.pay[dm.v[t]] = pay.v[t];
+          if (t < nd) gt.pay[t] = pay.v[t];
       }
     }
     return 0;
@@ -520,7 +515,7 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
     int64_t acc = a.goffset;
 #pragma unroll
     for (int t = 0; t < kMaxDims; t++)
-      if (t < nd) acc += (int64_t)pay.v[t] * a.dim[dm.v[t]].gmult;
+      if (t < nd) acc += (int64_t)pay.v[t] * a.dim[t].gmult;
     g = acc % a.n_groups;
   }
   const int n_bad = __popcll(__ballot(alive && g < 0));
@@ -535,7 +530,7 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
       const int32_t old = atomicCAS(&gt.key[slot], kEmptyGroup, (int32_t)g);
       if (old == kEmptyGroup) {
         for (int t = 0; t < kMaxDims; t++)
-          if (t < nd) gt.pay[slot * kMaxDims + dm.v[t]] = pay.v[t];
+          if (t < nd) gt.pay[slot * kMaxDims + t] = pay.v[t];
         k = (int32_t)g;
       } else {
         k = old;
@@ -549,7 +544,7 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
     }
     slot = (slot + 1) & smask;
   }
-  global_group_add(a, g, (long long)m, 1ull, p, pay, dm, nd);
+  global_group_add(a, g, (long long)m, 1ull, p, pay, nd);
   return n_bad;
 }
 
@@ -656,6 +651,12 @@ __device__ __forceinline__ void issue_batch(const ProbeArgs& a, WaveCtx& w, int 
   w.p_k = k;
 }
 
+// The value of dimension t from values kept by stage (stage s reads
+// dimension w.ld(s); the order is wave-uniform).
+__device__ __forceinline__ int32_t stage_to_dim(const WaveCtx& w, const Quad& by_stage, int t) {
+  return w.ld0 == t ? by_stage.v[0] : w.ld1 == t ? by_stage.v[1] : w.ld2 == t ? by_stage.v[2] : by_stage.v[3];
+}
+
 template <int K0, int SPEC>
 __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_t* lds, GroupLds& gt, WaveCtx& w,
                                                WaveStats& st) {
@@ -667,14 +668,13 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
   const int32_t row = w.p_row & 0x7FFFFFFF;
   const int first = done1 ? 2 : 1;
   // keys and payloads by stage (stage s reads dimension w.ld(s)): registers
-  // picked by compile-time stage numbers only
-  Quad key, pay, dm;
+  // picked by compile-time stage numbers only; the final table probes and the
+  // group add run by dimension (compile-time dimension numbers: the table
+  // arguments at constant offsets), each dimension's key picked once from the
+  // stages
+  Quad key, pay;
   key.clear();
   pay.clear();
-  dm.v[0] = w.ld0;
-  dm.v[1] = w.ld1;
-  dm.v[2] = w.ld2;
-  dm.v[3] = w.ld3;
   if (alive) {
     if (LIP) key.v[0] = w.p_x0;
     else pay.v[0] = w.p_x0;
@@ -709,6 +709,12 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
     }
   }
   int32_t m = 0;
+  Quad pd;  // payloads by dimension
+  pd.clear();
+  if (!LIP) {
+#pragma unroll
+    for (int t = 0; t < kMaxDims; t++) pd.v[t] = stage_to_dim(w, pay, t);
+  }
   if (LIP) {
     const unsigned long long bloom_pass = __ballot(alive);
     st.v[ST_BLOOM_PASS] += __popcll(bloom_pass);
@@ -724,7 +730,7 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
         alive = alive && w.p_ht != 0;
         p = (int32_t)(uint32_t)(w.p_ht >> 32);
       }
-      pay.v[0] = p;
+      pd.v[0] = p;
       m = w.p_m;
     } else if (bloom_pass != 0) {
       // After the Bloom chain every key is known: the PHT_2 probes of all
@@ -733,16 +739,19 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
       // row, or a key outside the table, reads slot 0 / row 0 and drops it),
       // so no branch separates them.  A row is alive iff every probe hits, as
       // with the probes taken one after another in join order.
-      // (every stage's slot is loaded, a stage past nd from stage 0's table:
-      // no branch on nd between the loads either)
+      // (every slot is loaded, the one of a dimension past nd from dimension
+      // 0's table: no branch on nd between the loads either)
+      Quad kd;
+#pragma unroll
+      for (int t = 0; t < kMaxDims; t++) kd.v[t] = stage_to_dim(w, key, t);
       uint64_t slot[kMaxDims];
       int64_t sls[kMaxDims];
       bool hit = alive;
 #pragma unroll
       for (int t = 0; t < kMaxDims; t++) {
         const bool on = t < nd;
-        const ProbeDim& D = a.dim[on ? dm.v[t] : w.ld0];
-        const int64_t sl = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, key.v[t]);
+        const ProbeDim& D = a.dim[on ? t : 0];
+        const int64_t sl = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, kd.v[t]);
         hit = hit && (!on || sl >= 0);
         sls[t] = sl;
         slot[t] = ht_load(D.ht_kind, D.ht, alive && on && sl >= 0 ? sl : 0);
@@ -752,15 +761,15 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
 #pragma unroll
       for (int t = 0; t < kMaxDims; t++) {
         if (t >= nd) continue;
-        const ProbeDim& D = a.dim[dm.v[t]];
+        const ProbeDim& D = a.dim[t];
         int32_t p = 0;
         if (D.ht_kind == HT_TEXTBOOK) {
-          if (hit) hit = ht_resolve(D.ht_kind, D.ht, D.ht_len, key.v[t], sls[t], slot[t], p);
+          if (hit) hit = ht_resolve(D.ht_kind, D.ht, D.ht_len, kd.v[t], sls[t], slot[t], p);
         } else {
           hit = hit && slot[t] != 0;
           p = (int32_t)(uint32_t)(slot[t] >> 32);
         }
-        pay.v[t] = p;
+        pd.v[t] = p;
       }
       alive = hit;
     }
@@ -771,7 +780,7 @@ __device__ __forceinline__ void complete_batch(const ProbeArgs& a, const uint32_
     m = nd == 1 ? w.p_m : a.measure[(uint32_t)row];
     if (nd > 1 && a.measure_sub) m = (int32_t)((uint32_t)m - (uint32_t)a.measure_sub[(uint32_t)row]);
   }
-  st.v[ST_OOB] += group_add(a, gt, alive, pay, dm, m);
+  st.v[ST_OOB] += group_add(a, gt, alive, pd, m);
   w.p_cnt = 0;
 }
 
@@ -843,11 +852,23 @@ __device__ __forceinline__ uint32_t dense_stage1(const ProbeArgs& a, const uint3
 
 // Append the alive rows of one segment to the wave queue (row order kept).
 // done1: the rows passed stage 1 too (x1 holds its key / payload).
+// The four ballots of a lane mask of 4 rows (bit j: row j), computed once and
+// shared by the row count and the queue positions.
+struct Ballot4 {
+  unsigned long long b[4];
+  __device__ __forceinline__ explicit Ballot4(uint32_t m) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) b[j] = __ballot((m >> j) & 1u);
+  }
+  __device__ __forceinline__ int count() const {
+    return __popcll(b[0]) + __popcll(b[1]) + __popcll(b[2]) + __popcll(b[3]);
+  }
+  __device__ __forceinline__ uint32_t below() const { return mbcnt(b[0]) + mbcnt(b[1]) + mbcnt(b[2]) + mbcnt(b[3]); }
+};
+
 __device__ __forceinline__ void enqueue(WaveCtx& w, uint32_t row0, const int32_t (&x0)[4], const int32_t (&x1)[4],
-                                        uint32_t mask, bool done1, int cnt_total) {
-  const unsigned long long b0 = __ballot(mask & 1u), b1 = __ballot(mask & 2u), b2 = __ballot(mask & 4u),
-                           b3 = __ballot(mask & 8u);
-  uint32_t pos = (uint32_t)w.qcount + mbcnt(b0) + mbcnt(b1) + mbcnt(b2) + mbcnt(b3);
+                                        uint32_t mask, bool done1, int cnt_total, const Ballot4& bal) {
+  uint32_t pos = (uint32_t)w.qcount + bal.below();
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     if ((mask >> j) & 1u) {
@@ -858,6 +879,11 @@ __device__ __forceinline__ void enqueue(WaveCtx& w, uint32_t row0, const int32_t
     }
   }
   w.qcount += cnt_total;
+}
+
+__device__ __forceinline__ void enqueue(WaveCtx& w, uint32_t row0, const int32_t (&x0)[4], const int32_t (&x1)[4],
+                                        uint32_t mask, bool done1, int cnt_total) {
+  enqueue(w, row0, x0, x1, mask, done1, cnt_total, Ballot4(mask));
 }
 
 // One tile of a column into registers: unchecked 16-byte non-temporal loads
@@ -1008,10 +1034,18 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
 #pragma unroll
       for (int j = 0; j < 4; j++) valid4 |= (uint32_t)(row0 + j < tc.n) << j;
     }
-    // keys below keys_min: counted here (wave-wide, on the scalar unit)
+    // keys below keys_min: counted here (wave-wide, on the scalar unit); the
+    // rows are counted one by one only when some lane's smallest key is below
     const int32_t km = s0.kmin;
-    st.v[ST_OOB] += __popcll(__ballot((valid4 & 1u) && k[v].x < km)) + __popcll(__ballot((valid4 & 2u) && k[v].y < km)) +
-                    __popcll(__ballot((valid4 & 4u) && k[v].z < km)) + __popcll(__ballot((valid4 & 8u) && k[v].w < km));
+    int32_t kmin4 = min(min(k[v].x, k[v].y), min(k[v].z, k[v].w));
+    if (CHECKED) {
+      const int32_t big = INT32_MAX;
+      kmin4 = min(min((valid4 & 1u) ? k[v].x : big, (valid4 & 2u) ? k[v].y : big),
+                  min((valid4 & 4u) ? k[v].z : big, (valid4 & 8u) ? k[v].w : big));
+    }
+    if (__ballot(kmin4 < km))
+      st.v[ST_OOB] += __popcll(__ballot((valid4 & 1u) && k[v].x < km)) + __popcll(__ballot((valid4 & 2u) && k[v].y < km)) +
+                      __popcll(__ballot((valid4 & 4u) && k[v].z < km)) + __popcll(__ballot((valid4 & 8u) && k[v].w < km));
     mask[v] = s0_probe4<K0, SPEC != 1>(s0, k[v], valid4, pay[v]);
   }
   const uint32_t nvalid = CHECKED ? (tc.n - base < TILE ? tc.n - base : TILE) : TILE;
@@ -1034,7 +1068,8 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
         for (int j = 0; j < 4; j++) pv[j] = pay[u][j];
       }
     }
-    const int cnt = popc_ballot4(m);
+    const Ballot4 bal(m);
+    const int cnt = bal.count();
     total += cnt;
     if (cnt == 0) continue;
     // callers keep qcount < kQueue between tiles; later segments of a tile re-check
@@ -1077,7 +1112,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
       continue;
     }
     // a sparse segment adds at most kDenseMin rows: it fits (qcount < kQueue here)
-    enqueue(w, row0, qx0, qx1, m, false, cnt);
+    enqueue(w, row0, qx0, qx1, m, false, cnt, bal);
   }
   return total;
 }
@@ -1493,12 +1528,9 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   for (int i = threadIdx.x; i < gslots; i += WG) {
     const int32_t g = gt.key[i];
     if (g != kEmptyGroup && gt.cnt[i] != 0) {
-      Quad pay, ident;
-      for (int d = 0; d < kMaxDims; d++) {
-        pay.v[d] = gt.pay[i * kMaxDims + d];
-        ident.v[d] = d;
-      }
-      global_group_add(a, g, (long long)gt.sum[i], gt.cnt[i], gt.prod[i], pay, ident, a.n_dims);
+      Quad pay;
+      for (int d = 0; d < kMaxDims; d++) pay.v[d] = gt.pay[i * kMaxDims + d];
+      global_group_add(a, g, (long long)gt.sum[i], gt.cnt[i], gt.prod[i], pay, a.n_dims);
     }
   }
   for (int i = threadIdx.x; i < kStats; i += WG) a.stats[(int64_t)bid * kStats + i] = lds_stats[i];
