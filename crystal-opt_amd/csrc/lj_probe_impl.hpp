@@ -1218,14 +1218,15 @@ struct TileSched {
   uint32_t stride;   // waves in the grid
   uint32_t wg_base;  // b * NW
   // Issue the take; the value is read (readfirstlane) by tile() later, so the
-  // LDS atomic's latency hides behind the tile in flight.
+  // LDS atomic's latency hides behind the tile in flight.  ds_append adds the
+  // number of active lanes to the counter and returns its value before to
+  // every lane: called with the whole wave active (uniform code only), one
+  // take advances it by 64, with no exec masking or lane count around it.
   __device__ __forceinline__ uint32_t issue() const {
-    uint32_t j = 0;
-    if (lane_id() == 0) j = atomicAdd(next, 1u);
-    return j;
+    return (uint32_t)__builtin_amdgcn_ds_append((__attribute__((address_space(3))) int*)next);
   }
   __device__ __forceinline__ uint32_t tile(uint32_t raw) const {
-    const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)raw) + NW;
+    const uint32_t j = ((uint32_t)__builtin_amdgcn_readfirstlane((int)raw) >> 6) + NW;
     return (j / NW) * stride + wg_base + (j % NW);
   }
 };
