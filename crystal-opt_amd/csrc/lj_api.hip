@@ -2268,18 +2268,9 @@ static lj_status serve_launch(lj_server* S, ServeSlot& sl, std::vector<std::pair
       kinds[k].push_back(i);
       klds[k] = std::max(klds[k], sl.reqs[i].p.lds_bytes);
     }
-    // several stage-0 kinds at the default shape: one launch of the mixed-kind probe
-    bool one_shape = kinds.size() > 1;
-    for (const ProbeConfig& cf : kcfg) one_shape = one_shape && cf.wg == 1024 && cf.vec == 1;
-    if (one_shape) {
-      size_t lds = 0;
-      for (size_t x : klds) lds = std::max(lds, x);
-      std::vector<size_t> all;
-      for (size_t i = 0; i < sl.reqs.size(); i++) all.push_back(i);
-      kinds.assign(1, all);
-      kcfg.assign(1, ProbeConfig{1024, 1, kS0Mixed});
-      klds.assign(1, lds);
-    }
+    // (one launch per kind: an admitted group is one or two requests at the
+    // rates where latency matters, and the per-kind kernels are faster than
+    // the mixed-kind one)
     ProbeArgs* hq = reinterpret_cast<ProbeArgs*>(sl.args_host);
     uint32_t* hw = reinterpret_cast<uint32_t*>(sl.args_host + sl.wgb_off);
     std::vector<size_t> q0(kinds.size()), w0(kinds.size());
