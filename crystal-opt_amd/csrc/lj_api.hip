@@ -2393,12 +2393,7 @@ static void serve_loop(lj_server* S) {
       }
       if (slot) {
         const uint64_t budget = S->cfg.hbm_budget;
-        // an idle device takes the oldest request alone, so that it finishes
-        // as soon as it would one at a time; the requests behind it form the
-        // next launch, queued while it runs (batching pays once the device is
-        // busy, and only delays the first request of an idle device)
-        const int cap = any_busy ? S->cfg.max_batch : 1;
-        while (!S->pending.empty() && (int)take.size() < cap) {
+        while (!S->pending.empty() && (int)take.size() < S->cfg.max_batch) {
           ServeReq& rq = S->pending.front();
           if (budget && rq.bytes > budget) {  // can never be admitted
             serve_finish(S, rq, LJ_EBUDGET, "request needs more device memory than the serving budget", nullptr,
