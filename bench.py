@@ -107,10 +107,12 @@ def bench_batch(ctx, n_inst, reps):
             f"-2^{int(ch['bloom_bits']).bit_length() - 1}{'-lds' if ch['in_lds'] else ''}"
         kinds[k] = kinds.get(k, 0) + 1
     out["bloom_choices"] = kinds
+    out["tune_model"] = ctx.tune_model()
     out["separate_rows_per_s"] = rows / sep_s
     log(f"batch: {n_inst} joins, {rows / 1e9:.2f} G rows: separate {sep_s * 1e3:.2f} ms, "
         f"eager {out['batch_eager_ms']:.2f} ms, graph {out['batch_graph_ms']:.2f} ms, "
-        f"a launch per kind {out['batch_eager_per_kind_ms']:.2f} ms; filters {kinds}")
+        f"a launch per kind {out['batch_eager_per_kind_ms']:.2f} ms; filters {kinds}; "
+        f"tuner model {out['tune_model']}")
     del cols
     return out
 

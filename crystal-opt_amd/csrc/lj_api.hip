@@ -100,6 +100,8 @@ struct lj_ctx {
   int lds_policy = -1;
   int group_slots = 0;           // LJ_GROUP_SLOTS: the LDS group table's size, 0 = from the group space
   int filter_prefix = 1;         // LJ_FILTER_PREFIX=0: stage whole modulo filters
+  int partial_prefix = 1;        // LJ_PARTIAL_PREFIX=0: never stage part of a filter that does not fit
+  int exact_eager_div = 32;      // LJ_EXACT_EAGER: an exact semi-join streams its measure above 1/this pass
   double clock_khz = 100000.0;  // device wall clock (kernel phase timestamps)
   // Bloom autotuner cost model, per fact row, relative to the scan of an LDS filter
   // cost model of the Bloom autotuner, per fact row, in units of a scan with
@@ -353,6 +355,7 @@ extern "C" int64_t lj_abi_struct_size(int32_t which) {
     case 9: return sizeof(lj_bloom_choice);
     case 10: return sizeof(lj_serve_cfg);
     case 11: return sizeof(lj_serve_stats);
+    case 12: return sizeof(lj_tune_model);
   }
   return -1;
 }
@@ -430,6 +433,8 @@ extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* 
   c->lds_policy = env_int("LJ_LDS_POLICY", -1);
   c->group_slots = env_int("LJ_GROUP_SLOTS", 0);
   c->filter_prefix = env_int("LJ_FILTER_PREFIX", 1);
+  c->partial_prefix = env_int("LJ_PARTIAL_PREFIX", 1);
+  c->exact_eager_div = std::max(1, env_int("LJ_EXACT_EAGER", 32));
   // lj_star_join_noalloc's scratch on this device: its address, once per
   // device (the device is current), and the one clear of its zero region
   if (device < kNaMaxDevices) {
@@ -1000,6 +1005,33 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
       }
     }
   }
+  // A one-dimension semi-join through an exact filter (P.exact) that does
+  // not fit in LDS: its first words, as many as the LDS holds, are staged,
+  // and the probe reads a word past them from the global filter (the bodies
+  // that fall back to global words: all_k0 is cleared below).  Such a probe
+  // is bound by the rate of L2 requests of its filter reads, and the staged
+  // share of them leaves that stream (2^28 fact rows over a 2^22-bit filter:
+  // 1.38 -> 0.99 ms).  A join that probes the table for each survivor is
+  // bound by those gathers instead, and the generic body makes it slower
+  // (2.40 -> 2.92 ms): it keeps the whole filter in global memory.  Only
+  // when the fact rows repay each workgroup's staging copy kPrefixRepay
+  // times over: the probes moved to LDS, n * staged / words, against the
+  // copy, a 128-byte line per 32 staged words on each of about n_cu
+  // workgroups.
+  bool partial = false;
+  if (lip && !adaptive && nd == 1 && c->partial_prefix && lds_policy == 0 && a.dim[0].exact) {
+    ProbeDim& P = a.dim[0];
+    const bool mod = P.bh.kind == BLOOM_MOD || P.bh.kind == BLOOM_MOD_POW2;
+    const int64_t pw = lds_budget_words & ~3ll;
+    constexpr double kPrefixRepay = 4.0;
+    if (mod && P.bloom && P.bloom_lds < 0 && pw >= 1024 && pw < P.bloom_words &&
+        (double)q->n_fact >= kPrefixRepay * (double)c->n_cu * (double)P.bloom_words / 32.0) {
+      P.bloom_lds = 0;
+      P.bloom_lds_words = (int32_t)pw;
+      lds_words = (int)pw;
+      partial = true;
+    }
+  }
   // stage-0 kernel specialisation: the kind of the first filter (every
   // candidate's kind for the adaptive order; GENERIC when they differ)
   auto kind_of = [&](int d) -> int {
@@ -1045,6 +1077,7 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   a.all_k0 = lip && family(cfg.k0) != 0 ? 1 : 0;
   for (int d = 0; d < nd; d++)
     if (lip && family(kind_of(d)) != family(cfg.k0)) a.all_k0 = 0;
+  if (partial) a.all_k0 = 0;
   // the LDS group table: 32 slots, or for a small group space up to twice
   // its size (at most kGroupSlotsMax) in the LDS that staging leaves.  A
   // 1024-thread workgroup is alone on its CU (the probe's registers allow 16
@@ -1070,6 +1103,7 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   a.adaptive = adaptive ? 1 : 0;
   a.want_product = want_product;
   a.k0 = cfg.k0;
+  a.exact_eager_div = c->exact_eager_div;
   for (int i = 0; i < LJ_MAX_DIMS; i++) {
     a.order[i] = i < nd ? q->order[i] : i;
     a.load_order[i] = i < nd ? (lip ? p.bloom_order[i] : q->order[i]) : i;
@@ -1917,12 +1951,18 @@ extern "C" lj_status lj_tune_bloom(lj_ctx* c, int64_t n_selected, int64_t n_dim_
     for (const Cand& cd : cands) {
       const bool in_lds = (int64_t)m <= lds_bits;
       const double fpr = fpr_of(cd.hash, cd.layout, cd.k, (double)n_selected, m, (double)n_dim_rows);
-      const double unit = in_lds ? 1.0 : m <= (double)kTuneL2Bits ? c->tune_scan_gl : c->tune_scan_big;
+      // a semi-join's exact modulo filter decides the table probe: no gathers
+      const bool exact = semi && cd.hash == LJ_BLOOM_MOD && m >= (double)n_dim_rows;
+      double unit = in_lds ? 1.0 : m <= (double)kTuneL2Bits ? c->tune_scan_gl : c->tune_scan_big;
+      // and, too large for LDS, it is staged in part (plan_probe): that share
+      // of its probes costs what an LDS filter's do
+      if (exact && !in_lds && c->partial_prefix) {
+        const double f = (double)lds_bits / m;
+        unit = f + (1.0 - f) * unit;
+      }
       double scan = unit;
       if (cd.layout == LJ_BLOOM_STANDARD && cd.k > 1) scan += (cd.k - 1) * c->tune_probe_extra * unit;
       const double pass = fact_pass + (1.0 - fact_pass) * fpr;
-      // a semi-join's exact modulo filter decides the table probe: no gathers
-      const bool exact = semi && cd.hash == LJ_BLOOM_MOD && m >= (double)n_dim_rows;
       const double cost = scan + lines_of(pass) * c->tune_line + (exact ? 0.0 : pass * c->tune_gather);
       if (cost < best.cost * (1.0 - 1e-9)) {
         best.bloom_hash = cd.hash;
@@ -1936,6 +1976,34 @@ extern "C" lj_status lj_tune_bloom(lj_ctx* c, int64_t n_selected, int64_t n_dim_
     }
   }
   *out = best;
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_tune_get_model(const lj_ctx* c, lj_tune_model* out) {
+  if (!c || !out) return LJ_EINVAL;
+  *out = lj_tune_model{};
+  out->scan_l2 = c->tune_scan_gl;
+  out->scan_big = c->tune_scan_big;
+  out->probe_extra = c->tune_probe_extra;
+  out->line = c->tune_line;
+  out->gather = c->tune_gather;
+  out->calibrated = c->tune_calibrated;
+  return LJ_OK;
+}
+
+extern "C" lj_status lj_tune_set_model(lj_ctx* c, const lj_tune_model* in) {
+  if (!c || !in) return fail(c, LJ_EINVAL, "NULL argument");
+  auto finite = [](double x) { return x == x && x < 1e300 && x > -1e300; };
+  if (!finite(in->scan_l2) || !finite(in->scan_big) || !finite(in->probe_extra) || !finite(in->line) ||
+      !finite(in->gather) || in->scan_l2 < 1.0 || in->scan_big < in->scan_l2 || in->probe_extra < 0.0 ||
+      in->line < 0.0 || in->gather < 0.0)
+    return fail(c, LJ_EINVAL, "tune model: scan_l2 >= 1, scan_big >= scan_l2, the others >= 0, all finite");
+  c->tune_scan_gl = in->scan_l2;
+  c->tune_scan_big = in->scan_big;
+  c->tune_probe_extra = in->probe_extra;
+  c->tune_line = in->line;
+  c->tune_gather = in->gather;
+  c->tune_calibrated = 1;
   return LJ_OK;
 }
 

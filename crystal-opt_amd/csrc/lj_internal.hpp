@@ -134,7 +134,8 @@ struct ProbeArgs {
   int32_t g_reps;
   int32_t check_past;      // some dimension passes past_count: the probe reads it at its start
   int32_t k0;              // the query's stage-0 kind (S0Kind; read by probe_mixed_kernel)
-  int32_t reserved_k0;
+  int32_t exact_eager_div;  // an exact one-dimension semi-join streams its measure with the keys when
+                            // stage 0 passes more than 1 / exact_eager_div of its first tile's rows
   // if set (a single query): the header and the first host_groups groups are
   // also written to this pinned host memory, so no copy follows the probe
   long long* host_hdr;

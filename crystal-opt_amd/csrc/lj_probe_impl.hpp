@@ -1482,7 +1482,10 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       const int tot = tile_step<VEC, K0, true, SPEC>(a, s0, lds_filter, gt, w, st, tc, base, k, ke, false, oob);
       if (w.qcount >= kQueue) rotate_batch<K0, SPEC>(a, lds_filter, gt, w, st);
       const uint32_t nv = tc.n - base < TILE ? tc.n - base : TILE;
-      if (pass == 0 && !a.adaptive) eager = (uint32_t)tot * 4 > nv;
+      // (the exact semi-join finishes a streamed segment in place, so it
+      // streams from a lower pass rate than the queue's gathers would pay)
+      if (pass == 0 && !a.adaptive)
+        eager = (uint32_t)tot * (SPEC == 3 && w.exact ? (uint32_t)a.exact_eager_div : 4u) > nv;
     }
     if (pass == 0) {
       TileSched<NW> sch;

@@ -261,6 +261,17 @@ class Context:
     def tune_calibrate(self):
         self._check(lib().lj_tune_calibrate(self.h))
 
+    def tune_model(self):
+        """lj_tune_get_model: the tuner's cost constants (a dict)."""
+        m = _abi.TuneModel()
+        self._check(lib().lj_tune_get_model(self.h, C.byref(m)))
+        return {k: getattr(m, k) for k, _ in _abi.TuneModel._fields_ if k != "reserved"}
+
+    def set_tune_model(self, model):
+        """lj_tune_set_model from a dict like tune_model()'s."""
+        m = _abi.TuneModel(**{k: model[k] for k in ("scan_l2", "scan_big", "probe_extra", "line", "gather")})
+        self._check(lib().lj_tune_set_model(self.h, C.byref(m)))
+
     # -- synthetic columns -------------------------------------------------------------
     def gen_device(self, out, kind, seed, stream, lo, hi, first=0, n=None):
         n = out.numel() if n is None else n

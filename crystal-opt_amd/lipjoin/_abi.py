@@ -131,8 +131,13 @@ class ServeStats(C.Structure):
                 ("max_batch_seen", C.c_uint64), ("reserved", C.c_uint64)]
 
 
+class TuneModel(C.Structure):
+    _fields_ = [("scan_l2", C.c_double), ("scan_big", C.c_double), ("probe_extra", C.c_double),
+                ("line", C.c_double), ("gather", C.c_double), ("calibrated", C.c_int32), ("reserved", C.c_int32)]
+
+
 STRUCTS = [DimDesc, DimInfo, ProbeStats, JoinDesc, JoinResult, StarDesc, Group, StarResult, BatchOpts,
-           BloomChoice, ServeCfg, ServeStats]
+           BloomChoice, ServeCfg, ServeStats, TuneModel]
 
 # name -> (restype, argtypes); every entry point of include/lipjoin.h
 SIGNATURES = {
@@ -157,6 +162,8 @@ SIGNATURES = {
     "lj_batch_destroy": (None, [C.c_void_p]),
     "lj_tune_bloom": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_int32, C.POINTER(BloomChoice)]),
     "lj_tune_calibrate": (C.c_int, [C.c_void_p]),
+    "lj_tune_get_model": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lj_tune_set_model": (C.c_int, [C.c_void_p, C.c_void_p]),
     "lj_serve_start": (C.c_int, [C.c_void_p, C.POINTER(ServeCfg), C.POINTER(C.c_void_p)]),
     "lj_serve_submit": (C.c_int, [C.c_void_p, C.POINTER(StarDesc), C.POINTER(C.c_uint64)]),
     "lj_serve_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(StarResult), C.c_int32]),

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define LJ_ABI_VERSION 4
+#define LJ_ABI_VERSION 5
 #define LJ_MAX_DIMS 4
 #define LJ_MAX_PRED_STEPS 4
 
@@ -133,7 +133,9 @@ const char* lj_status_str(lj_status s);
 int32_t lj_abi_version(void);
 /* sizeof of the public structs, for binding checks: 0 lj_dim_desc, 1 lj_dim_info,
  * 2 lj_probe_stats, 3 lj_join_desc, 4 lj_join_result, 5 lj_star_desc,
- * 6 lj_group, 7 lj_star_result; -1 for an unknown index. */
+ * 6 lj_group, 7 lj_star_result, 8 lj_batch_opts, 9 lj_bloom_choice,
+ * 10 lj_serve_cfg, 11 lj_serve_stats, 12 lj_tune_model; -1 for an unknown
+ * index. */
 int64_t lj_abi_struct_size(int32_t which);
 /* Wait for all work queued by the context. */
 lj_status lj_sync(lj_ctx* ctx);
@@ -401,6 +403,21 @@ lj_status lj_tune_bloom(lj_ctx* ctx, int64_t n_selected, int64_t n_dim_rows, dou
 /* Measures the model's constants on this device (a few probes of synthetic
  * data, about a second) and keeps them in the context. */
 lj_status lj_tune_calibrate(lj_ctx* ctx);
+/* The model's constants, per fact row in units of a scan with the filter in
+ * LDS.  Read them after lj_tune_calibrate to keep a measurement; set them to
+ * reuse it on another context of the same device (no counterpart in the
+ * reference, whose Bloom sizes are fixed by its drivers' arguments). */
+typedef struct {
+  double scan_l2;          /* scan with the first filter in L2 (up to 2 MiB), >= 1 */
+  double scan_big;         /* scan with a larger filter, >= scan_l2 */
+  double probe_extra;      /* each further bit probe of a standard k > 1 filter, >= 0 */
+  double line;             /* a 128-byte line of the next column that holds a passing row, >= 0 */
+  double gather;           /* a passing row (queue, hash-slot gather), >= 0 */
+  int32_t calibrated;      /* get: 1 once measured or set */
+  int32_t reserved;
+} lj_tune_model;
+lj_status lj_tune_get_model(const lj_ctx* ctx, lj_tune_model* out);
+lj_status lj_tune_set_model(lj_ctx* ctx, const lj_tune_model* in);
 
 /* ---- seeded synthetic columns ------------------------------------------------ */
 /* Counter-based generator: value j = first + i of stream `stream` depends only

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match():
     assert _abi.check_layout() == []
-    assert _abi.load().lj_abi_version() == 4
+    assert _abi.load().lj_abi_version() == 5
 
 
 def test_status_strings():
@@ -52,3 +52,10 @@ def test_host_generator_argument_checks():
     lib = _abi.load()
     assert lib.lj_gen_host(0, 1, 1, 5, 4, 0, 10, None) == _abi.EINVAL     # hi < lo
     assert lib.lj_gen_host(9, 1, 1, 0, 4, 0, 10, None) == _abi.EINVAL     # unknown kind
+
+
+def test_tune_model_argument_checks():
+    lib = _abi.load()
+    m = _abi.TuneModel(scan_l2=2.0, scan_big=3.0, probe_extra=0.2, line=1.0, gather=6.0)
+    assert lib.lj_tune_set_model(None, C.byref(m)) == _abi.EINVAL
+    assert lib.lj_tune_get_model(None, C.byref(m)) == _abi.EINVAL

@@ -123,3 +123,69 @@ def test_tuner_semi_join_prefers_exact_filter(ctx):
     dimension's rows in bits) wins at a high pass rate."""
     ch = ctx.tune_bloom(1 << 21, 1 << 22, 0.5, semi_join=True)
     assert ch["bloom_hash"] == L.BLOOM_MOD and ch["bloom_bits"] >= 1 << 22 and ch["fpr"] == 0.0
+
+
+# ---- a filter too large for LDS, staged in part ------------------------------------
+# A one-dimension semi-join whose exact modulo filter does not fit in LDS
+# stages the filter's first words (what the LDS holds, about 1.2 Mbit) when
+# the fact rows repay the copy (n_fact >= 4 x CUs x filter words / 32: 4.2 M
+# rows for a 2^22-bit filter on 256 CUs); the probe reads the words past them
+# from the global filter.  2^23 fact rows over a 2^22-key dimension take that
+# path; a context with LJ_PARTIAL_PREFIX=0 reads the whole filter from global
+# memory.  The joins with payloads (not sum-only) keep the whole filter in
+# global memory either way: they check that the rule leaves them alone.
+N_BIG = 1 << 22
+
+
+def _big_case(seed, thr, wrap_dim=False):
+    rng = np.random.default_rng(seed)
+    keys = np.arange(N_BIG, dtype=np.int32)
+    if wrap_dim:
+        keys[-700:] = N_BIG + 11 + np.arange(700, dtype=np.int32) * 5
+    attr = rng.integers(0, 100, N_BIG, dtype=np.int32)
+    attr[0] = 0
+    n = (1 << 23) + 19
+    fk = rng.integers(0, N_BIG, n, dtype=np.int32)
+    fk[rng.integers(0, n, 5000)] = 0
+    far = rng.integers(0, n, 6000)
+    fk[far] = rng.integers(N_BIG, 3 * N_BIG, far.size, dtype=np.int32)
+    meas = rng.integers(1, 10001, n, dtype=np.int32)
+    return keys, attr, fk, meas
+
+
+BIG_CASES = [
+    # (threshold, filter bits, sum only, dimension keys past the domain)
+    (50, 1 << 22, True, False),      # exact filter, streamed measure
+    (5, 1 << 22, True, False),       # exact filter, queued survivors
+    (50, 1 << 22, True, True),       # aliased dimension keys: the table decides
+    (50, 1 << 22, False, False),     # a filtered join with payloads: not staged
+    (20, 3_000_017, False, False),   # a modulo filter of fewer bits than keys: not exact, not staged
+    (20, (1 << 22) + 4099, True, False),  # exact, words not a power of two
+]
+
+
+@pytest.mark.parametrize("thr,bits,sum_only,wrap", BIG_CASES,
+                         ids=[f"s{t}-b{b}-{'semi' if s else 'full'}-{'wrap' if w else 'in'}"
+                              for t, b, s, w in BIG_CASES])
+def test_partially_staged_filter_against_oracle(ctx, thr, bits, sum_only, wrap):
+    keys, attr, fk, meas = _big_case(4321 + thr + bits % 1000, thr, wrap)
+    _flush(f"partially staged filter thr={thr} bits={bits} sum_only={sum_only} wrap={wrap}")
+    pred = ((L.COMB_SET, L.CMP_LT, thr),)
+    dev = to_device({"keys": keys, "attr": attr, "fk": fk, "meas": meas})
+    desc = L.dim_desc(dev["keys"], N_BIG, N_BIG, attr=dev["attr"], predicate=pred, bloom=("mod", bits))
+    r = ctx.hash_join(dev["fk"], dev["meas"], fk.size, desc, method="lip", sum_only=sum_only)
+    spec = O.DimSpec(keys, attr, N_BIG, pred=list(pred), bloom=O.bloom_spec(O.BLOOM_MOD, bits, 1))
+    ref = O.single_join(spec, fk, meas, method=O.LIP, threads=os.cpu_count() or 1)
+    got = (r["n_selected"], r["n_bloom_pass"], r["n_match"], r["sum_measure"])
+    assert got == (ref["n_selected"], ref["n_bloom_pass"], ref["n_match"], ref["sum_measure"])
+    if not sum_only and not wrap:
+        assert r["sum_product"] == ref["sum_product"]
+    os.environ["LJ_PARTIAL_PREFIX"] = "0"
+    try:
+        with L.Context(0) as c2:
+            whole = c2.hash_join(dev["fk"], dev["meas"], fk.size, desc, method="lip", sum_only=sum_only)
+    finally:
+        del os.environ["LJ_PARTIAL_PREFIX"]
+    key = lambda x: (x["n_bloom_pass"], x["n_match"], x["sum_measure"], x["sum_product"], x["stats"]["lines_in"][:2])
+    assert key(r) == key(whole)
+    torch.cuda.synchronize()
