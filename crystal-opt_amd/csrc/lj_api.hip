@@ -57,6 +57,7 @@ struct lj_dim {
 };
 
 constexpr int kEvents = 4;
+constexpr int64_t kMaxGroupSpace = 1ll << 26;  // a query's group space (n_groups) at most
 constexpr int kEvDone = 1;
 
 struct lj_ctx {
@@ -854,7 +855,7 @@ static lj_status validate_query(lj_ctx* c, const lj_star_desc* q, Plan& p) {
   for (int i = 0; i < nd; i++) bo_default = bo_default && q->bloom_order[i] < 0;
   for (int i = 0; i < nd; i++) p.bloom_order[i] = bo_default ? q->order[i] : q->bloom_order[i];
   if (!is_perm(p.bloom_order, nd)) return fail(c, LJ_EINVAL, "bloom_order is not a permutation");
-  if (q->n_groups < 1 || q->n_groups > (1ll << 26)) return fail(c, LJ_EINVAL, "n_groups %lld not in [1, 2^26]", (long long)q->n_groups);
+  if (q->n_groups < 1 || q->n_groups > kMaxGroupSpace) return fail(c, LJ_EINVAL, "n_groups %lld not in [1, 2^26]", (long long)q->n_groups);
   if (q->n_fact > 0) {
     if (!q->fact_measure || ((uintptr_t)q->fact_measure & 15)) return fail(c, LJ_EINVAL, "fact_measure must be a 16-byte aligned device pointer");
     if ((uintptr_t)q->fact_measure_sub & 15) return fail(c, LJ_EINVAL, "fact_measure_sub must be 16-byte aligned");
@@ -2208,7 +2209,7 @@ static uint64_t serve_bytes(const lj_server* S, int64_t n_groups) {
 
 static void reqbuf_free(lj_ctx* c, ReqBuf* b) {
   dev_release(c, b->dev);
-  if (b->host) (void)hipHostFree(b->host);
+  host_release(c, b->host, b->host_bytes);
   delete b;
 }
 
@@ -2238,9 +2239,8 @@ static lj_status reqbuf_get(lj_server* S, int64_t n_groups, ReqBuf** out) {
   b->host_bytes = kHdrBytes + sizeof(GroupOut) * (size_t)b->tcap;
   void* h = nullptr;
   void* hd = nullptr;
-  if (hipHostMalloc(&h, b->host_bytes, hipHostMallocDefault) != hipSuccess ||
-      hipHostGetDevicePointer(&hd, h, 0) != hipSuccess) {
-    if (h) (void)hipHostFree(h);
+  if (host_alloc(c, b->host_bytes, &h) != LJ_OK || hipHostGetDevicePointer(&hd, h, 0) != hipSuccess) {
+    host_release(c, h, b->host_bytes);
     b->host = nullptr;
     reqbuf_free(c, b);
     return fail(c, LJ_EHIP, "pinned result buffer allocation failed");
@@ -2529,6 +2529,61 @@ static void server_free(lj_server* S) {
   delete S;
 }
 
+// One launch of each batched probe kernel (the 1024-thread kinds the loop
+// launches) on an empty query: no fact rows, one workgroup, the buffers of
+// a one-group request.  A kernel's first launch in a process spends
+// milliseconds loading it (6.9 ms measured), which the first requests of
+// that kind in a batched launch, and every request queued behind them,
+// would otherwise wait for.
+static lj_status serve_warm(lj_server* S) {
+  lj_ctx* c = S->c;
+  ServeSlot& sl = S->slots[0];
+  ReqBuf* b = nullptr;
+  lj_status s = reqbuf_get(S, 1, &b);
+  if (s != LJ_OK) return s;
+  auto done = [&](lj_status st) {
+    (void)hipStreamSynchronize(sl.stream);
+    S->free_bufs.emplace(b->ng, b);
+    return st;
+  };
+  if (b->fresh) {
+    FillArgs f{};
+    add_fill(f, b->dev, (int64_t)(b->l.zero / 4));
+    if (launch_fill(f, sl.stream) != hipSuccess) return done(fail(c, LJ_EHIP, "fill launch failed"));
+    b->fresh = false;
+  }
+  ProbeArgs a{};
+  qbuf_bind(a, b->ng, b->tcap, b->dev, b->dev + b->l.zero, b->dev + b->l.zero + b->l.plain);
+  a.n = 0;
+  a.n_dims = 1;
+  a.n_groups = 1;
+  a.gslot_bits = ilog2_u64((uint64_t)kGroupSlots);
+  a.exact_eager_div = c->exact_eager_div;
+  for (int i = 0; i < LJ_MAX_DIMS; i++) {
+    a.order[i] = a.load_order[i] = i;
+    a.dim[i].bloom_lds = -1;
+  }
+  ProbeArgs* hq = reinterpret_cast<ProbeArgs*>(sl.args_host);
+  uint32_t* hw = reinterpret_cast<uint32_t*>(sl.args_host + sl.wgb_off);
+  hq[0] = a;
+  hw[0] = 0;
+  hw[1] = 1;
+  if (hipMemcpyAsync(sl.args_dev, sl.args_host, sl.args_bytes, hipMemcpyHostToDevice, sl.stream) != hipSuccess)
+    return done(fail(c, LJ_EHIP, "hipMemcpyAsync failed"));
+  const ProbeArgs* dq = reinterpret_cast<const ProbeArgs*>(sl.args_dev);
+  const uint32_t* dw = reinterpret_cast<const uint32_t*>(sl.args_dev + sl.wgb_off);
+  for (int k0 = 0; k0 < S0_NKINDS; k0++) {
+    const ProbeConfig cfg{1024, 1, k0};
+    if (!probe_has_entry(cfg)) continue;
+    const size_t lds = probe_lds_bytes(cfg, 0, kGroupSlots);
+    int per_cu = 0;
+    if ((s = occupancy_of(c, cfg, lds, &per_cu)) != LJ_OK) return done(s);
+    if (launch_probe_multi(cfg, dq, dw, 1, 1, lds, sl.stream) != hipSuccess)
+      return done(fail(c, LJ_EHIP, "warm-up launch failed"));
+  }
+  return done(hipStreamSynchronize(sl.stream) == hipSuccess ? LJ_OK : fail(c, LJ_EHIP, "warm-up failed"));
+}
+
 extern "C" lj_status lj_serve_start(lj_ctx* c, const lj_serve_cfg* cfg, lj_server** out) {
   if (!c || !out) return fail(c, LJ_EINVAL, "NULL argument");
   *out = nullptr;
@@ -2542,6 +2597,11 @@ extern "C" lj_status lj_serve_start(lj_ctx* c, const lj_serve_cfg* cfg, lj_serve
   if (S->cfg.max_batch > 4096 || S->cfg.max_inflight > 8) {
     delete S;
     return fail(c, LJ_EINVAL, "max_batch <= 4096 and max_inflight <= 8");
+  }
+  if (S->cfg.prewarm_groups < 0 || S->cfg.prewarm_groups > kMaxGroupSpace) {
+    delete S;
+    return fail(c, LJ_EINVAL, "prewarm_groups %lld outside [0, %lld]", (long long)S->cfg.prewarm_groups,
+                (long long)kMaxGroupSpace);
   }
   if (S->cfg.streams <= 0) S->cfg.streams = 1;
   S->cfg.streams = std::min(S->cfg.streams, S->cfg.max_inflight);
@@ -2574,7 +2634,43 @@ extern "C" lj_status lj_serve_start(lj_ctx* c, const lj_serve_cfg* cfg, lj_serve
       return s;
     }
   }
+  // request buffers of the announced group-space class, allocated and
+  // cleared now: the loop then takes them from its free list instead of
+  // allocating device and pinned host memory on the first requests (a
+  // millisecond or more each, which every request behind them waits for)
+  if (S->cfg.prewarm_groups > 0) {
+    // (no more than the serving budget can hold in flight at once)
+    int64_t nb_warm = (int64_t)S->cfg.max_inflight * S->cfg.max_batch;
+    if (S->cfg.hbm_budget)
+      nb_warm = std::min<int64_t>(nb_warm, (int64_t)(S->cfg.hbm_budget / serve_bytes(S, S->cfg.prewarm_groups)));
+    std::vector<ReqBuf*> warm;
+    FillArgs f{};
+    lj_status s = LJ_OK;
+    for (int64_t i = 0; i < nb_warm && s == LJ_OK; i++) {
+      ReqBuf* b = nullptr;
+      s = reqbuf_get(S, S->cfg.prewarm_groups, &b);
+      if (s != LJ_OK) break;
+      warm.push_back(b);
+      if (f.n == kMaxFill) {
+        if (launch_fill(f, c->stream) != hipSuccess) s = fail(c, LJ_EHIP, "fill launch failed");
+        f = FillArgs{};
+      }
+      add_fill(f, b->dev, (int64_t)(b->l.zero / 4));
+      b->fresh = false;
+    }
+    if (s == LJ_OK && f.n && launch_fill(f, c->stream) != hipSuccess) s = fail(c, LJ_EHIP, "fill launch failed");
+    for (ReqBuf* b : warm) S->free_bufs.emplace(b->ng, b);
+    if (s != LJ_OK) {
+      (void)hipStreamSynchronize(c->stream);
+      server_free(S);
+      return s;
+    }
+  }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (lj_status s = serve_warm(S); s != LJ_OK) {
+    server_free(S);
+    return s;
+  }
   S->th = std::thread(serve_loop, S);
   *out = S;
   return LJ_OK;

@@ -220,6 +220,7 @@ int probe_entries_generic(const ProbeEntry** out);
 
 size_t probe_lds_bytes(const ProbeConfig& c, int lds_filter_words, int group_slots);
 hipError_t probe_occupancy(const ProbeConfig& c, size_t lds_bytes, int* blocks_per_cu);
+bool probe_has_entry(const ProbeConfig& c);  // a kernel is built for this configuration
 // A batch: qs[n_q] and wg_begin[n_q + 1] in device memory; grid == wg_begin[n_q].
 hipError_t launch_probe_multi(const ProbeConfig& c, const ProbeArgs* qs, const uint32_t* wg_begin, int n_q, int grid,
                               size_t lds_bytes, hipStream_t s);

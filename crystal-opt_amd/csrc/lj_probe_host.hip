@@ -29,6 +29,8 @@ static const ProbeEntry* find_entry(const ProbeConfig& c) {
   return nullptr;
 }
 
+bool probe_has_entry(const ProbeConfig& c) { return find_entry(c) != nullptr; }
+
 static const void* find_probe(const ProbeConfig& c) {
   const ProbeEntry* e = find_entry(c);
   return e ? e->fn : nullptr;

@@ -245,8 +245,10 @@ class Context:
         return Batch(self, descs, use_graph, groups_cap, probe_wg, launch_per_kind)
 
     # -- serving (request loop over resident dimensions) ----------------------------------
-    def serve(self, hbm_budget=0, max_batch=64, max_inflight=2, groups_cap=64, streams=1):
-        return Server(self, hbm_budget, max_batch, max_inflight, groups_cap, streams)
+    def serve(self, hbm_budget=0, max_batch=64, max_inflight=2, groups_cap=64, streams=1, prewarm_groups=0):
+        """lj_serve_start; prewarm_groups > 0 allocates the request buffers for
+        queries of up to that many groups at start."""
+        return Server(self, hbm_budget, max_batch, max_inflight, groups_cap, streams, prewarm_groups)
 
     # -- Bloom autotuner ------------------------------------------------------------------
     def tune_bloom(self, n_selected, n_dim_rows, fact_pass, semi_join=False):
@@ -338,12 +340,14 @@ class Server:
     """lj_server: submit star joins over resident dimensions from any thread,
     poll for results.  The context must not be used otherwise until stop()."""
 
-    def __init__(self, ctx, hbm_budget=0, max_batch=64, max_inflight=2, groups_cap=64, streams=1):
+    def __init__(self, ctx, hbm_budget=0, max_batch=64, max_inflight=2, groups_cap=64, streams=1,
+                 prewarm_groups=0):
         self.ctx = ctx
         cfg = _abi.ServeCfg()
         cfg.hbm_budget, cfg.max_batch, cfg.max_inflight, cfg.groups_cap = (int(hbm_budget), int(max_batch),
                                                                           int(max_inflight), int(groups_cap))
         cfg.streams = int(streams)
+        cfg.prewarm_groups = int(prewarm_groups)
         self.cap = int(groups_cap)
         h = C.c_void_p()
         ctx._check(lib().lj_serve_start(ctx.h, C.byref(cfg), C.byref(h)))

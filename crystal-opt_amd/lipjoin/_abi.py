@@ -122,7 +122,7 @@ class BloomChoice(C.Structure):
 class ServeCfg(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
                 ("groups_cap", C.c_int64), ("streams", C.c_int32), ("reserved0", C.c_int32),
-                ("reserved1", C.c_int64)]
+                ("prewarm_groups", C.c_int64)]
 
 
 class ServeStats(C.Structure):

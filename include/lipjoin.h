@@ -342,7 +342,9 @@ typedef struct {
                               compete for the device's caches); k > 1: slot i runs on
                               stream i % k and launches overlap on the device */
   int32_t reserved0;
-  int64_t reserved1;
+  int64_t prewarm_groups;  /* > 0: at start, allocate and clear max_inflight * max_batch
+                              request buffers for queries of up to this many groups, so
+                              the first requests do not wait for allocations; 0: none */
 } lj_serve_cfg;
 
 typedef struct {

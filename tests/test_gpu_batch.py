@@ -150,6 +150,25 @@ def test_tune_calibrate(ctx):
     assert ch["bloom_bits"] >= 1 << 16 and 0.0 <= ch["fpr"] < 1.0
 
 
+def test_tune_model_get_set():
+    """A calibrated model carried to another context gives the same choices;
+    a model outside its bounds is refused and leaves the context's as it was."""
+    with L.Context(0) as a, L.Context(0) as b:
+        a.tune_calibrate()
+        m = a.tune_model()
+        assert m["calibrated"] == 1 and m["scan_l2"] >= 1.0 and m["scan_big"] >= m["scan_l2"]
+        b.set_tune_model(m)
+        got = b.tune_model()
+        assert all(got[k] == m[k] for k in ("scan_l2", "scan_big", "probe_extra", "line", "gather"))
+        for args in [(200000, 1 << 22, 0.05, False), (1 << 21, 1 << 22, 0.3, True), (5000, 1 << 20, 0.01, False)]:
+            assert a.tune_bloom(*args[:3], semi_join=args[3]) == b.tune_bloom(*args[:3], semi_join=args[3])
+        for bad in [dict(m, scan_l2=0.5), dict(m, scan_big=m["scan_l2"] - 0.1), dict(m, gather=-1.0),
+                    dict(m, line=float("nan"))]:
+            with pytest.raises(L.LJError):
+                b.set_tune_model(bad)
+        assert b.tune_model() == got
+
+
 def test_tune_bloom_loaded_filters(ctx):
     # half of 2^22 keys selected: a filter with fewer bits than keys passes
     # nearly every row, so it must not be chosen (its modelled FPR is ~1)

@@ -157,3 +157,23 @@ def test_serve_config5_shape_against_oracle(ctx):
             (ref["n_match"], ref["sum_measure"], ref["n_bloom_pass"]), (i, t, o, n)
     for tm in tmpls:
         tm["dim"].release()
+
+
+@pytest.mark.parametrize("warm", [1, 1000])
+def test_serve_prewarmed_buffers(ctx, serve_setup, warm):
+    """Request buffers allocated and cleared at start (prewarm_groups): the
+    requests of that class take them, others allocate as before; every result
+    is the same as alone, and a group space outside [0, 2^26] is refused."""
+    cols, tmpls, pool, reqs, descs, alone = serve_setup
+    _flush(f"serve prewarm {warm}")
+    srv = ctx.serve(hbm_budget=1 << 30, max_batch=8, max_inflight=2, prewarm_groups=warm)
+    ids = [srv.submit(d) for d in descs]
+    out = [srv.poll(i, timeout_ms=60000) for i in ids]
+    srv.stop()
+    assert all(o is not None and o["status"] == L._abi.OK for o in out)
+    for i, (o, a) in enumerate(zip(out, alone)):
+        assert _key(o) == _key(a), i
+    with pytest.raises(L.LJError):
+        ctx.serve(prewarm_groups=-1)
+    with pytest.raises(L.LJError):
+        ctx.serve(prewarm_groups=(1 << 26) + 1)
