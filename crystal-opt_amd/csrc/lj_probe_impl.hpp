@@ -45,6 +45,8 @@
 
 namespace lj {
 
+constexpr int kStageUnroll = 8;     // 16-byte filter loads in flight per thread while staging
+constexpr int kFinUnroll = 8;       // statistics rows per thread read together by the finalize
 constexpr int kDenseMin = 8;         // stage-0 survivors of a segment above which it is processed in place
 constexpr int kGroupProbes = 32;     // slots a row tries in the LDS group table before going global
 static_assert(kGroupSlotBytes == 3 * 8 + 4 + kMaxDims * 4, "group slot layout");
@@ -919,19 +921,43 @@ __device__ __forceinline__ void finalize_block(const ProbeArgs& a, long long* pa
   // the kernel ends.
   const uint32_t n = __hip_atomic_load(a.n_touched, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t g_first = threadIdx.x < a.touched_cap ? a.touched[threadIdx.x] : 0u;
+  // the statistics rows: the first kFinUnroll rows of each column are read
+  // together (one round trip for a grid of up to kFinUnroll * ROWS
+  // workgroups), issued before the counters' loads; the rest after
+  const int k = threadIdx.x % kStats, r = threadIdx.x / kStats;
+  long long sv[kFinUnroll];
+#pragma unroll
+  for (int j = 0; j < kFinUnroll; j++) {
+    const int b = r + j * ROWS;
+    sv[j] = k != ST_ORDER && b < (int)nblk ? a.stats[(int64_t)b * kStats + k] : 0;
+  }
   long long dimc = 0;
   if (threadIdx.x < 2 * kMaxDims) {
+    // the counter block's address: a select over the four (uniform) fields,
+    // not a per-lane load of the argument array
     const int d = threadIdx.x / 2;
-    if (a.dim_counters[d])
-      for (int l = 0; l < kCounterLanes; l++) dimc += (long long)a.dim_counters[d][2 * l + threadIdx.x % 2];
+    static_assert(kMaxDims == 4, "dimension select");
+    auto sp = [](const unsigned long long* q) {  // in scalar registers
+      const uint64_t u = (uint64_t)(uintptr_t)q;
+      const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+      const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+      return (const unsigned long long*)(uintptr_t)(((uint64_t)hi << 32) | lo);
+    };
+    const unsigned long long *c0 = sp(a.dim_counters[0]), *c1 = sp(a.dim_counters[1]),
+                             *c2 = sp(a.dim_counters[2]), *c3 = sp(a.dim_counters[3]);
+    const unsigned long long* dc = d == 0 ? c0 : d == 1 ? c1 : d == 2 ? c2 : c3;
+    if (dc)
+      for (int l = 0; l < kCounterLanes; l++) dimc += (long long)gl(dc)[2 * l + threadIdx.x % 2];
   }
   {
-    const int k = threadIdx.x % kStats, r = threadIdx.x / kStats;
     long long acc = 0;
-    if (k != ST_ORDER)
-      for (int b = r; b < (int)nblk; b += ROWS) acc += a.stats[(int64_t)b * kStats + k];
-    else if (r == 0)
+    if (k != ST_ORDER) {
+#pragma unroll
+      for (int j = 0; j < kFinUnroll; j++) acc += sv[j];
+      for (int b = r + kFinUnroll * ROWS; b < (int)nblk; b += ROWS) acc += a.stats[(int64_t)b * kStats + k];
+    } else if (r == 0) {
       acc = a.stats[ST_ORDER];  // the order used, from workgroup 0
+    }
     part[r * kStats + k] = acc;
   }
   __syncthreads();  // `part` complete; every thread has read n_touched
@@ -1364,9 +1390,31 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       const int nw = D.bloom_lds_words;  // the staged prefix
       uint32_t* dst = lds_filter + D.bloom_lds;
       if (((D.bloom_lds | nw) & 3) == 0 && (((uintptr_t)D.bloom) & 15) == 0) {
-        const uint4* src4 = reinterpret_cast<const uint4*>(D.bloom);
-        uint4* dst4 = reinterpret_cast<uint4*>(dst);
-        for (int i = threadIdx.x; i < nw / 4; i += WG) dst4[i] = src4[i];
+        // kStageUnroll 16-byte loads in flight per thread before their LDS
+        // stores (a loop of load, wait, store pays one memory round trip per
+        // 16 KB of filter)
+        // (address-space-typed pointers: no flat accesses)
+        const __attribute__((address_space(1))) v4i32* src4 = gl(reinterpret_cast<const v4i32*>(D.bloom));
+        __attribute__((address_space(3))) v4i32* dst4 = (__attribute__((address_space(3))) v4i32*)dst;
+        const int n4 = nw / 4;
+        for (int i0 = 0; i0 < n4; i0 += WG * kStageUnroll) {
+          // Indexes past the end are clamped to the last word, which such
+          // lanes copy again (the same value): loads and stores stay
+          // unconditional, so the loads are not sunk into a branch each
+          // beside its store, and the barrier keeps them all ahead.
+          v4i32 v[kStageUnroll];
+#pragma unroll
+          for (int u = 0; u < kStageUnroll; u++) {
+            const int i = i0 + u * WG + (int)threadIdx.x;
+            v[u] = src4[i < n4 ? i : n4 - 1];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 0; u < kStageUnroll; u++) {
+            const int i = i0 + u * WG + (int)threadIdx.x;
+            dst4[i < n4 ? i : n4 - 1] = v[u];
+          }
+        }
       } else {
         for (int i = threadIdx.x; i < nw; i += WG) dst[i] = D.bloom[i];
       }
