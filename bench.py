@@ -302,7 +302,7 @@ def bench_serve(ctx, n_req, fractions=(0.25, 0.5, 0.75, 0.9, 1.1, 1.5, 2.0)):
         lam = f * capacity
         arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
         # serving loop
-        srv = ctx.serve(hbm_budget=200 << 30, max_batch=64, max_inflight=2, prewarm_groups=1)
+        srv = ctx.serve(hbm_budget=200 << 30, max_batch=64, max_inflight=4, prewarm_groups=1)
         lag = []
         t0 = time.perf_counter()
         ids = []

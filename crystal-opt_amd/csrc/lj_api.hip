@@ -2165,6 +2165,7 @@ struct ServeSlot {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   char* args_dev = nullptr;   // ProbeArgs [max_batch] | wg_begin [max_batch + S0_NKINDS]
   char* args_host = nullptr;  // pinned staging of args_dev
+  char* args_host_dev = nullptr;  // the same memory, as the device addresses it
   size_t args_bytes = 0;
   size_t wgb_off = 0;
   bool busy = false;
@@ -2280,6 +2281,22 @@ static void serve_finish(lj_server* S, const ServeReq& rq, lj_status st, const c
   S->done[rq.id] = std::move(d);
 }
 
+// The slot's first n_q ProbeArgs and n_w workgroup offsets, from its pinned
+// staging into its device buffer, ordered on the slot's stream (a copy
+// kernel: see copy_kernel).
+static hipError_t upload_args(const ServeSlot& sl, size_t n_q, size_t n_w) {
+  static_assert(sizeof(ProbeArgs) % 4 == 0, "ProbeArgs in words");
+  CopyArgs ca{};
+  ca.r[0].dst = reinterpret_cast<uint32_t*>(sl.args_dev);
+  ca.r[0].src = reinterpret_cast<const uint32_t*>(sl.args_host_dev);
+  ca.r[0].words = (int64_t)(n_q * sizeof(ProbeArgs) / 4);
+  ca.r[1].dst = reinterpret_cast<uint32_t*>(sl.args_dev + sl.wgb_off);
+  ca.r[1].src = reinterpret_cast<const uint32_t*>(sl.args_host_dev + sl.wgb_off);
+  ca.r[1].words = (int64_t)n_w;
+  ca.n = 2;
+  return launch_copy(ca, sl.stream);
+}
+
 // Plan the admitted requests and queue their launch on the slot's stream.
 // Requests that fail to plan finish at once with their own status (out).
 static lj_status serve_launch(lj_server* S, ServeSlot& sl, std::vector<std::pair<ServeReq, lj_status>>& failed) {
@@ -2363,7 +2380,7 @@ static lj_status serve_launch(lj_server* S, ServeSlot& sl, std::vector<std::pair
       hw[wi++] = begin;
       kgrid[k] = begin;
     }
-    HIP_TRY(c, hipMemcpyAsync(sl.args_dev, sl.args_host, sl.args_bytes, hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(c, upload_args(sl, qi, wi));
     const ProbeArgs* dq = reinterpret_cast<const ProbeArgs*>(sl.args_dev);
     const uint32_t* dw = reinterpret_cast<const uint32_t*>(sl.args_dev + sl.wgb_off);
     for (size_t k = 0; k < kinds.size(); k++)
@@ -2568,8 +2585,7 @@ static lj_status serve_warm(lj_server* S) {
   hq[0] = a;
   hw[0] = 0;
   hw[1] = 1;
-  if (hipMemcpyAsync(sl.args_dev, sl.args_host, sl.args_bytes, hipMemcpyHostToDevice, sl.stream) != hipSuccess)
-    return done(fail(c, LJ_EHIP, "hipMemcpyAsync failed"));
+  if (upload_args(sl, 1, 2) != hipSuccess) return done(fail(c, LJ_EHIP, "argument upload failed"));
   const ProbeArgs* dq = reinterpret_cast<const ProbeArgs*>(sl.args_dev);
   const uint32_t* dw = reinterpret_cast<const uint32_t*>(sl.args_dev + sl.wgb_off);
   for (int k0 = 0; k0 < S0_NKINDS; k0++) {
@@ -2592,7 +2608,7 @@ extern "C" lj_status lj_serve_start(lj_ctx* c, const lj_serve_cfg* cfg, lj_serve
   S->c = c;
   if (cfg) S->cfg = *cfg;
   if (S->cfg.max_batch <= 0) S->cfg.max_batch = 64;
-  if (S->cfg.max_inflight <= 0) S->cfg.max_inflight = 2;
+  if (S->cfg.max_inflight <= 0) S->cfg.max_inflight = 4;
   if (S->cfg.groups_cap <= 0) S->cfg.groups_cap = 64;
   if (S->cfg.max_batch > 4096 || S->cfg.max_inflight > 8) {
     delete S;
@@ -2625,7 +2641,8 @@ extern "C" lj_status lj_serve_start(lj_ctx* c, const lj_serve_cfg* cfg, lj_serve
     lj_status s = LJ_OK;
     if (hipEventCreateWithFlags(&sl.e0, hipEventDefault) != hipSuccess ||
         hipEventCreateWithFlags(&sl.e1, hipEventDefault) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&sl.args_host), sl.args_bytes, hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc(reinterpret_cast<void**>(&sl.args_host), sl.args_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&sl.args_host_dev), sl.args_host, 0) != hipSuccess)
       s = fail(c, LJ_EHIP, "serving slot setup failed");
     if (s == LJ_OK) s = dev_alloc(c, sl.args_bytes, &d);
     sl.args_dev = static_cast<char*>(d);

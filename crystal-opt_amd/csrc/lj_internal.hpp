@@ -175,8 +175,22 @@ struct GroupOut {
 };
 
 
+// Word copies on a stream (up to two ranges in one launch); a source may be
+// pinned host memory through its device address.
+struct CopyRange {
+  uint32_t* dst;
+  const uint32_t* src;
+  int64_t words;
+};
+struct CopyArgs {
+  CopyRange r[2];
+  int32_t n;
+  int32_t pad0;
+};
+
 // Host launchers (lj_kernels_*.hip).
 hipError_t launch_fill(const FillArgs& a, hipStream_t s);
+hipError_t launch_copy(const CopyArgs& a, hipStream_t s);
 hipError_t launch_build(BuildArgs& a, hipStream_t s);
 hipError_t launch_gen(int32_t kind, uint64_t key, int64_t lo, int64_t hi, int64_t first, int64_t n,
                       int32_t* out, hipStream_t s);

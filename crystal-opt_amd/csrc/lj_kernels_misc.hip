@@ -48,6 +48,29 @@ hipError_t launch_fill(const FillArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- copy ----
+// A kernel rather than hipMemcpyAsync for the serving loop's launch
+// arguments: the host-to-device copy engine path could block the calling
+// thread for milliseconds behind the stream's earlier work, and the sources
+// here are small (pinned host memory read through its device address).
+__global__ __launch_bounds__(256) void copy_kernel(CopyArgs a) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int r = 0; r < a.n; r++)
+    for (int64_t i = tid; i < a.r[r].words; i += stride) a.r[r].dst[i] = a.r[r].src[i];
+}
+
+hipError_t launch_copy(const CopyArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  int64_t maxw = 0;
+  for (int r = 0; r < a.n; r++) maxw = a.r[r].words > maxw ? a.r[r].words : maxw;
+  int64_t blocks = (maxw + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 64) blocks = 64;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // --------------------------------------------------------------- build ----
 // Predicate, Bloom insert, PHT_2 insert (CAS of the key, then the
 // unconditional value store; PHT_1: the CAS alone, join.cuh:191-258; or the

@@ -245,7 +245,7 @@ class Context:
         return Batch(self, descs, use_graph, groups_cap, probe_wg, launch_per_kind)
 
     # -- serving (request loop over resident dimensions) ----------------------------------
-    def serve(self, hbm_budget=0, max_batch=64, max_inflight=2, groups_cap=64, streams=1, prewarm_groups=0):
+    def serve(self, hbm_budget=0, max_batch=64, max_inflight=4, groups_cap=64, streams=1, prewarm_groups=0):
         """lj_serve_start; prewarm_groups > 0 allocates the request buffers for
         queries of up to that many groups at start."""
         return Server(self, hbm_budget, max_batch, max_inflight, groups_cap, streams, prewarm_groups)
@@ -340,7 +340,7 @@ class Server:
     """lj_server: submit star joins over resident dimensions from any thread,
     poll for results.  The context must not be used otherwise until stop()."""
 
-    def __init__(self, ctx, hbm_budget=0, max_batch=64, max_inflight=2, groups_cap=64, streams=1,
+    def __init__(self, ctx, hbm_budget=0, max_batch=64, max_inflight=4, groups_cap=64, streams=1,
                  prewarm_groups=0):
         self.ctx = ctx
         cfg = _abi.ServeCfg()

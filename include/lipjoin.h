@@ -335,7 +335,7 @@ typedef struct lj_server lj_server;
 typedef struct {
   uint64_t hbm_budget;     /* bytes of per-request device memory in flight; 0: unlimited */
   int32_t max_batch;       /* requests per launch; 0: 64 */
-  int32_t max_inflight;    /* launches in flight, each on its own stream; 0: 2 */
+  int32_t max_inflight;    /* launch slots (launches in flight); 0: 4 */
   int64_t groups_cap;      /* result groups kept per request; 0: 64 */
   int32_t streams;         /* streams the launch slots use: 0 or 1 = one stream, so a new
                               launch queues behind the one in flight (no two launches
