@@ -1530,6 +1530,40 @@ extern "C" lj_status lj_gen_reference_host(int64_t n_dim, int64_t n_fact, int32_
   return LJ_OK;
 }
 
+// The reference's star-schema inputs (src/ops/utils/star.cuh:67-120,
+// star_table_init_host): srand(0); the fact table's create_relation_pk (keys
+// 1..n_fact shuffled, values 0..n_fact-1); then per dimension, in order, its
+// create_relation_pk (keys 1..n_rows shuffled, values 0..n_rows-1) and the
+// fact's foreign keys by create_relation_fk_only (generator.h:139-165:
+// successive shuffled blocks of 1..n_rows and a shuffled tail).
+extern "C" lj_status lj_gen_reference_star_host(int64_t n_fact, int32_t n_dims, const int64_t* dim_rows,
+                                                int32_t* fact_keys, int32_t* fact_vals, int32_t* const* dim_keys,
+                                                int32_t* const* dim_vals, int32_t* const* fact_fks) {
+  if (n_fact < 0 || n_fact >= (1ll << 31) || n_dims < 0 || n_dims > LJ_MAX_DIMS || (n_dims > 0 && !dim_rows))
+    return fail(nullptr, LJ_EINVAL, "n_fact %lld / n_dims %d out of range", (long long)n_fact, n_dims);
+  for (int d = 0; d < n_dims; d++)
+    if (dim_rows[d] < 1 || dim_rows[d] >= (1ll << 31))
+      return fail(nullptr, LJ_EINVAL, "dimension %d: %lld rows", d, (long long)dim_rows[d]);
+  std::vector<int32_t> scratch;
+  srand(0);  // check_seed (generator.h:38-45)
+  ref_unique_shuffled(fact_keys, n_fact, scratch);
+  if (fact_vals)
+    for (int64_t i = 0; i < n_fact; i++) fact_vals[i] = (int32_t)i;
+  for (int d = 0; d < n_dims; d++) {
+    const int64_t nr = dim_rows[d];
+    int32_t* dk = dim_keys ? dim_keys[d] : nullptr;
+    int32_t* dv = dim_vals ? dim_vals[d] : nullptr;
+    int32_t* fk = fact_fks ? fact_fks[d] : nullptr;
+    ref_unique_shuffled(dk, nr, scratch);
+    if (dv)
+      for (int64_t i = 0; i < nr; i++) dv[i] = (int32_t)i;
+    const int64_t iters = n_fact / nr, rem = n_fact % nr;
+    for (int64_t i = 0; i < iters; i++) ref_unique_shuffled(fk ? fk + nr * i : nullptr, nr, scratch);
+    if (rem > 0) ref_unique_shuffled(fk ? fk + nr * iters : nullptr, rem, scratch);
+  }
+  return LJ_OK;
+}
+
 // ------------------------------------------------------------------- batches --
 // n queries in one build launch and one probe launch per stage-0 kernel kind;
 // query q's workgroups are [wg_begin[q], wg_begin[q+1]) of its kind's grid.

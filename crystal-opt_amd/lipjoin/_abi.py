@@ -175,6 +175,8 @@ SIGNATURES = {
                                 C.c_int64, C.c_int64, C.c_void_p]),
     "lj_gen_shuffle_host": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, C.c_void_p]),
     "lj_gen_reference_host": (C.c_int, [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lj_gen_reference_star_host": (C.c_int, [C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

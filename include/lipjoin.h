@@ -451,6 +451,17 @@ lj_status lj_gen_shuffle_host(uint64_t seed, uint64_t stream, int64_t n, int32_t
  * still drawn, so the others do not change). */
 lj_status lj_gen_reference_host(int64_t n_dim, int64_t n_fact, int32_t* dim_keys, int32_t* dim_vals,
                                 int32_t* fact_keys, int32_t* fact_vals);
+/* The reference's star-schema inputs (src/ops/utils/star.cuh:67-120,
+ * star_table_init_host, with the generators above): srand(0), the fact
+ * table's keys 1..n_fact shuffled and values 0..n_fact-1, then for each of
+ * n_dims dimensions in turn its keys 1..dim_rows[d] shuffled, its values
+ * 0..dim_rows[d]-1 and the fact's foreign keys into it (shuffled blocks of
+ * 1..dim_rows[d] and a shuffled tail, create_relation_fk_only).  Any array,
+ * or entry of dim_keys / dim_vals / fact_fks, may be NULL (its values are
+ * still drawn).  Uses the C library's rand() (not thread-safe). */
+lj_status lj_gen_reference_star_host(int64_t n_fact, int32_t n_dims, const int64_t* dim_rows,
+                                     int32_t* fact_keys, int32_t* fact_vals, int32_t* const* dim_keys,
+                                     int32_t* const* dim_vals, int32_t* const* fact_fks);
 
 #ifdef __cplusplus
 }
