@@ -287,8 +287,10 @@ static lj_status dev_alloc_zero(lj_ctx* c, size_t bytes, void** out) {
   return LJ_OK;
 }
 
-// Queue the clear of a block and park it in the zero pool.
-static void dev_release_zero(lj_ctx* c, void* p, FillArgs& f) {
+// Queue the clear of a block and park it in the zero pool.  used_bytes: the
+// prefix its user could have written (the block came zero from the pool, so
+// the rest of it still is); 0 = the whole block.
+static void dev_release_zero(lj_ctx* c, void* p, FillArgs& f, size_t used_bytes = 0) {
   if (!p) return;
   auto it = c->live.find(p);
   if (it == c->live.end()) return;
@@ -296,8 +298,9 @@ static void dev_release_zero(lj_ctx* c, void* p, FillArgs& f) {
     (void)launch_fill(f, c->stream);
     f.n = 0;
   }
+  const size_t clear = used_bytes && used_bytes < it->second ? used_bytes : it->second;
   f.r[f.n].ptr = static_cast<uint32_t*>(p);
-  f.r[f.n].words = (int64_t)(it->second / 4);
+  f.r[f.n].words = (int64_t)((clear + 3) / 4);
   f.r[f.n].value = 0;
   f.n++;
   c->zero_blocks.emplace(it->second, p);
@@ -562,9 +565,9 @@ static void dims_free(lj_ctx* c, lj_dim* const* dims, int n) {
     lj_dim* D = dims[i];
     if (!D) continue;
     if (D->zero_pool) {
-      dev_release_zero(c, D->ht, f);
-      dev_release_zero(c, D->bloom, f);
-      dev_release_zero(c, D->counters, f);
+      dev_release_zero(c, D->ht, f, (size_t)D->ht_words * 4);
+      dev_release_zero(c, D->bloom, f, (size_t)D->bloom_words * 4);
+      dev_release_zero(c, D->counters, f, (size_t)kCounterWords * 8);
     } else {
       dev_release(c, D->ht);
       dev_release(c, D->bloom);
