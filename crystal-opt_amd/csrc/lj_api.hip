@@ -107,11 +107,16 @@ struct lj_ctx {
   // Bloom autotuner cost model, per fact row, relative to the scan of an LDS filter
   // cost model of the Bloom autotuner, per fact row, in units of a scan with
   // the filter in LDS (defaults: typical values lj_tune_calibrate measures)
-  double tune_scan_gl = 3.0;     // scan with the first filter in L2 instead of LDS (up to kTuneL2Bits)
-  double tune_scan_big = 4.0;    // scan with a larger filter (L2 misses, served from the MALL)
-  double tune_probe_extra = 0.2; // each further bit probe of a standard k > 1 filter
-  double tune_line = 1.0;        // a 128-byte line of the next column that holds a passing row
-  double tune_gather = 6.0;      // a row that passes the filter (queue, hash-slot gather)
+  double tune_scan_gl = 2.7;     // scan with the first filter in L2 instead of LDS (up to kTuneL2Bits)
+  double tune_scan_big = 2.7;    // scan with a larger filter (L2 misses, served from the MALL)
+  double tune_scan_mod = 0.45;   // scan with a modulo filter in LDS
+  double tune_scan_blk = 0.65;   // scan with a blocked multiplicative filter in LDS
+  double tune_probe_extra = 0.23;// each further bit probe of a standard k > 1 filter outside LDS (x scan)
+  double tune_line = 0.36;       // a 128-byte line of the next column that holds a passing row
+  double tune_gather = 5.7;      // a passing row when more than a quarter pass (the measure streams), L2 filter
+  double tune_gather_lds = 9.3;  // the same with the filter in LDS (nothing else to overlap the gathers with)
+  double tune_gather_q = 5.3;    // a passing row when fewer pass (queue, hash-slot and measure gathers), L2 filter
+  double tune_gather_q_lds = 15.0;  // the same with the filter in LDS (its short scan hides no gather)
   int tune_calibrated = 0;
   std::map<std::pair<int, size_t>, int> occupancy;  // (config key, LDS bytes) -> workgroups per CU
   std::multimap<size_t, void*> pinned_free;         // pinned host blocks for reuse (batches)
@@ -1991,17 +1996,37 @@ extern "C" lj_status lj_tune_bloom(lj_ctx* c, int64_t n_selected, int64_t n_dim_
       const double fpr = fpr_of(cd.hash, cd.layout, cd.k, (double)n_selected, m, (double)n_dim_rows);
       // a semi-join's exact modulo filter decides the table probe: no gathers
       const bool exact = semi && cd.hash == LJ_BLOOM_MOD && m >= (double)n_dim_rows;
-      double unit = in_lds ? 1.0 : m <= (double)kTuneL2Bits ? c->tune_scan_gl : c->tune_scan_big;
+      // in units of a scan with a standard multiplicative filter in LDS
+      const double lds_unit = cd.hash == LJ_BLOOM_MOD ? c->tune_scan_mod
+                              : cd.layout == LJ_BLOOM_BLOCKED ? c->tune_scan_blk : 1.0;
+      double unit = in_lds ? lds_unit : m <= (double)kTuneL2Bits ? c->tune_scan_gl : c->tune_scan_big;
       // and, too large for LDS, it is staged in part (plan_probe): that share
       // of its probes costs what an LDS filter's do
       if (exact && !in_lds && c->partial_prefix) {
         const double f = (double)lds_bits / m;
-        unit = f + (1.0 - f) * unit;
+        unit = f * c->tune_scan_mod + (1.0 - f) * unit;
       }
+      // (further bit probes of a standard filter cost nothing measurable in
+      // LDS; outside it each is another L2 request for the rows still alive)
       double scan = unit;
-      if (cd.layout == LJ_BLOOM_STANDARD && cd.k > 1) scan += (cd.k - 1) * c->tune_probe_extra * unit;
+      if (cd.layout == LJ_BLOOM_STANDARD && cd.k > 1 && !in_lds) scan += (cd.k - 1) * c->tune_probe_extra * unit;
       const double pass = fact_pass + (1.0 - fact_pass) * fpr;
-      const double cost = scan + lines_of(pass) * c->tune_line + (exact ? 0.0 : pass * c->tune_gather);
+      // The probe streams the measure with the keys when more than a quarter
+      // of a tile's rows pass: the cost per passing row is its hash-slot
+      // gather (smaller beside an L2 filter's scan, which it overlaps).
+      // Otherwise the passing rows are queued and gather the measure too.
+      // Each wave decides on its first tile, so between 20 % and 30 % the
+      // two costs are blended (no step for a filter to slip across).
+      double cost;
+      if (exact) {
+        cost = scan + lines_of(pass) * c->tune_line;
+      } else {
+        const double streamed = scan + pass * (in_lds ? c->tune_gather_lds : c->tune_gather);
+        const double queued = scan + lines_of(pass) * c->tune_line + pass * (in_lds ? c->tune_gather_q_lds
+                                                                                    : c->tune_gather_q);
+        const double w = std::min(1.0, std::max(0.0, (pass - 0.2) / 0.1));
+        cost = (1.0 - w) * queued + w * streamed;
+      }
       if (cost < best.cost * (1.0 - 1e-9)) {
         best.bloom_hash = cd.hash;
         best.bloom_layout = cd.layout;
@@ -2025,6 +2050,11 @@ extern "C" lj_status lj_tune_get_model(const lj_ctx* c, lj_tune_model* out) {
   out->probe_extra = c->tune_probe_extra;
   out->line = c->tune_line;
   out->gather = c->tune_gather;
+  out->scan_mod = c->tune_scan_mod;
+  out->scan_blocked = c->tune_scan_blk;
+  out->gather_queue = c->tune_gather_q;
+  out->gather_lds = c->tune_gather_lds;
+  out->gather_queue_lds = c->tune_gather_q_lds;
   out->calibrated = c->tune_calibrated;
   return LJ_OK;
 }
@@ -2033,24 +2063,41 @@ extern "C" lj_status lj_tune_set_model(lj_ctx* c, const lj_tune_model* in) {
   if (!c || !in) return fail(c, LJ_EINVAL, "NULL argument");
   auto finite = [](double x) { return x == x && x < 1e300 && x > -1e300; };
   if (!finite(in->scan_l2) || !finite(in->scan_big) || !finite(in->probe_extra) || !finite(in->line) ||
-      !finite(in->gather) || in->scan_l2 < 1.0 || in->scan_big < in->scan_l2 || in->probe_extra < 0.0 ||
-      in->line < 0.0 || in->gather < 0.0)
-    return fail(c, LJ_EINVAL, "tune model: scan_l2 >= 1, scan_big >= scan_l2, the others >= 0, all finite");
+      !finite(in->gather) || !finite(in->scan_mod) || !finite(in->scan_blocked) || !finite(in->gather_queue) ||
+      !finite(in->gather_lds) || !finite(in->gather_queue_lds) || in->scan_l2 < 1.0 ||
+      in->scan_big < in->scan_l2 || in->probe_extra < 0.0 || in->line < 0.0 || in->gather < 0.0 ||
+      in->gather_queue < 0.0 || in->gather_lds < 0.0 || in->gather_queue_lds < 0.0 ||
+      !(in->scan_mod > 0.0) || !(in->scan_blocked > 0.0))
+    return fail(c, LJ_EINVAL,
+                "tune model: scan_l2 >= 1, scan_big >= scan_l2, scan_mod and scan_blocked > 0, the others >= 0, "
+                "all finite");
   c->tune_scan_gl = in->scan_l2;
   c->tune_scan_big = in->scan_big;
   c->tune_probe_extra = in->probe_extra;
   c->tune_line = in->line;
   c->tune_gather = in->gather;
+  c->tune_scan_mod = in->scan_mod;
+  c->tune_scan_blk = in->scan_blocked;
+  c->tune_gather_q = in->gather_queue;
+  c->tune_gather_lds = in->gather_lds;
+  c->tune_gather_q_lds = in->gather_queue_lds;
   c->tune_calibrated = 1;
   return LJ_OK;
 }
 
+// (a calibration run's filter: the blocked multiplicative layout, k = 2)
+constexpr int kCalBlocked = 100;
+
 // Calibration: a dimension of 2^22 dense keys and a fact table of 2^25 rows
-// (device-generated), five LIP joins, each timed over all rows and over a
-// quarter of them (their difference is the cost per row): a filter in LDS
-// that nothing passes, a 2^22-bit (L2) and a 2^25-bit filter that nothing
-// passes, and the exact 2^22-bit filter that 1 % and that 40 % pass (their
-// survivor costs are measured against the empty filter of the same size).
+// (device-generated), nine LIP joins, each timed over all rows and over a
+// quarter of them (their difference is the cost per row): 2^20-bit filters in
+// LDS that nothing passes (standard multiplicative, the unit; modulo;
+// blocked), a 2^22-bit (L2) and a 2^25-bit filter that nothing passes, the
+// exact 2^22-bit filter that 20 % and 40 % pass, and the 2^20-bit modulo
+// filter in LDS over 1 % and 10 % of the keys (4 keys a bit: about 5 % and
+// 41 % pass; survivor costs are measured against the empty filter of the
+// same kind).
+
 extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
   if (!c) return fail(c, LJ_EINVAL, "NULL context");
   HIP_TRY(c, hipSetDevice(c->device));
@@ -2100,7 +2147,11 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
     d.ht_kind = LJ_HT_PHT2;
     d.ht_len = nd;
     d.bloom_hash = hash;
-    d.bloom_k = 1;
+    d.bloom_k = hash == kCalBlocked ? 2 : 1;
+    if (hash == kCalBlocked) {
+      d.bloom_hash = LJ_BLOOM_MULT;
+      d.bloom_layout = LJ_BLOOM_BLOCKED;
+    }
     d.bloom_bits = bits;
     q.n_groups = 1;
     float best = 1e30f;
@@ -2122,30 +2173,50 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
     *per_row = (t_all - t_quarter) / (0.75f * (float)nf);
     return st;
   };
-  float t_lds = 0, t_l2 = 0, t_big = 0, t_p1 = 0, t_p40 = 0;
+  float t_lds = 0, t_mod = 0, t_blk = 0, t_l2 = 0, t_big = 0, t_p20 = 0, t_p40 = 0, t_m5 = 0, t_m41 = 0;
   // the survivor costs are measured with every passing row reading the table
   // (these one-group joins would otherwise let the exact filter decide)
   const int exact_saved = c->exact_filter;
   c->exact_filter = 0;
-  if ((s = run(LJ_BLOOM_MULT, 1 << 20, 0, &t_lds)) != LJ_OK || (s = run(LJ_BLOOM_MOD, nd, 0, &t_l2)) != LJ_OK ||
-      (s = run(LJ_BLOOM_MOD, 1 << 25, 0, &t_big)) != LJ_OK || (s = run(LJ_BLOOM_MOD, nd, 100, &t_p1)) != LJ_OK ||
-      (s = run(LJ_BLOOM_MOD, nd, 4000, &t_p40)) != LJ_OK) {
+  if ((s = run(LJ_BLOOM_MULT, 1 << 20, 0, &t_lds)) != LJ_OK || (s = run(LJ_BLOOM_MOD, 1 << 20, 0, &t_mod)) != LJ_OK ||
+      (s = run(kCalBlocked, 1 << 20, 0, &t_blk)) != LJ_OK || (s = run(LJ_BLOOM_MOD, nd, 0, &t_l2)) != LJ_OK ||
+      (s = run(LJ_BLOOM_MOD, 1 << 25, 0, &t_big)) != LJ_OK || (s = run(LJ_BLOOM_MOD, nd, 2000, &t_p20)) != LJ_OK ||
+      (s = run(LJ_BLOOM_MOD, nd, 4000, &t_p40)) != LJ_OK ||
+      (s = run(LJ_BLOOM_MOD, 1 << 20, 100, &t_m5)) != LJ_OK || (s = run(LJ_BLOOM_MOD, 1 << 20, 1000, &t_m41)) != LJ_OK) {
     c->exact_filter = exact_saved;
     release();
     return s;
   }
   c->exact_filter = exact_saved;
   release();
-  if (!(t_lds > 0 && t_l2 > 0 && t_big > 0)) return fail(c, LJ_EHIP, "calibration timings are not positive");
+  if (!(t_lds > 0 && t_mod > 0 && t_blk > 0 && t_l2 > 0 && t_big > 0))
+    return fail(c, LJ_EHIP, "calibration timings are not positive");
   c->tune_scan_gl = std::max(1.0, (double)t_l2 / t_lds);
   c->tune_scan_big = std::max(c->tune_scan_gl, (double)t_big / t_lds);
-  // t_p1 / t_p40 scan the same 2^22-bit (512 KiB) filter as t_l2 and pass 1 % /
-  // 40 % of the rows: e(p) = lines_of(p) * line + p * gather, two equations
-  const double e1 = std::max(0.0, ((double)t_p1 - t_l2) / t_lds), e40 = std::max(0.0, ((double)t_p40 - t_l2) / t_lds);
-  const double a1 = lines_of(0.01), a40 = lines_of(0.4), det = a1 * 0.4 - a40 * 0.01;
-  const double line = (e1 * 0.4 - e40 * 0.01) / det, gather = (a1 * e40 - a40 * e1) / det;
-  c->tune_line = std::max(0.0, line);
-  c->tune_gather = std::max(1.0, gather);
+  c->tune_scan_mod = std::min(1.0, (double)t_mod / t_lds);
+  c->tune_scan_blk = std::min(1.0, (double)t_blk / t_lds);
+  // t_p20 / t_p40 scan the same 2^22-bit (512 KiB) filter as t_l2 and pass
+  // 20 % and 40 % of the rows.  Up to a quarter the survivors' measure is
+  // gathered: e(p) = lines_of(p) * line + p * gather_q gives gather_q at 20 %
+  // (line stays the model's: the lines term barely changes between the pass
+  // rates a short calibration can time, so it cannot be fitted from them;
+  // 0.36 fits a 2^28-row sweep from 1 % to 19 %), and gather_q_lds from the
+  // LDS filter that passes about 5 %.  Above a quarter the measure streams
+  // with the keys: gather from 40 %, gather_lds from the LDS filter that
+  // passes about 41 %.
+  const double e20 = std::max(0.0, ((double)t_p20 - t_l2) / t_lds);
+  const double e40 = std::max(0.0, ((double)t_p40 - t_l2) / t_lds);
+  const double gather_q = (e20 - lines_of(0.20) * c->tune_line) / 0.20;
+  const auto p_mod20 = [&](double sel) {  // the 2^20-bit modulo filter over sel of the keys
+    return sel + (1.0 - sel) * fpr_of(LJ_BLOOM_MOD, LJ_BLOOM_STANDARD, 1, sel * (double)nd, (double)(1 << 20), (double)nd);
+  };
+  const double p5 = p_mod20(0.01), p41 = p_mod20(0.1);
+  const double e5 = std::max(0.0, ((double)t_m5 - t_mod) / t_lds);
+  const double e41 = std::max(0.0, ((double)t_m41 - t_mod) / t_lds);
+  c->tune_gather_q = std::max(1.0, gather_q);
+  c->tune_gather_q_lds = std::max(1.0, (e5 - lines_of(p5) * c->tune_line) / p5);
+  c->tune_gather = std::max(1.0, e40 / 0.4);
+  c->tune_gather_lds = std::max(1.0, e41 / p41);
   c->tune_calibrated = 1;
   return LJ_OK;
 }

@@ -271,7 +271,9 @@ class Context:
 
     def set_tune_model(self, model):
         """lj_tune_set_model from a dict like tune_model()'s."""
-        m = _abi.TuneModel(**{k: model[k] for k in ("scan_l2", "scan_big", "probe_extra", "line", "gather")})
+        m = _abi.TuneModel(**{k: model[k] for k in ("scan_l2", "scan_big", "probe_extra", "line", "gather",
+                                                     "scan_mod", "scan_blocked", "gather_queue", "gather_lds",
+                                                     "gather_queue_lds")})
         self._check(lib().lj_tune_set_model(self.h, C.byref(m)))
 
     # -- synthetic columns -------------------------------------------------------------

@@ -133,7 +133,10 @@ class ServeStats(C.Structure):
 
 class TuneModel(C.Structure):
     _fields_ = [("scan_l2", C.c_double), ("scan_big", C.c_double), ("probe_extra", C.c_double),
-                ("line", C.c_double), ("gather", C.c_double), ("calibrated", C.c_int32), ("reserved", C.c_int32)]
+                ("line", C.c_double), ("gather", C.c_double), ("scan_mod", C.c_double),
+                ("scan_blocked", C.c_double), ("gather_queue", C.c_double), ("gather_lds", C.c_double), ("gather_queue_lds", C.c_double),
+                ("calibrated", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 STRUCTS = [DimDesc, DimInfo, ProbeStats, JoinDesc, JoinResult, StarDesc, Group, StarResult, BatchOpts,

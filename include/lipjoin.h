@@ -380,8 +380,12 @@ lj_status lj_serve_stop(lj_server* srv);
  * Picks the Bloom filter (hash, layout, k, bits) of a LIP join on one
  * dimension: n_selected keys pass the dimension predicate, the fact side has
  * n_fact rows of which a fraction fact_pass joins.  The cost model per fact
- * row is scan(bits, k, where the filter lives: LDS, L2 up to 2 MiB, or
- * beyond) + lines(P) * line + P * gather, with
+ * row is scan(hash, layout, k, where the filter lives: LDS, L2 up to 2 MiB,
+ * or beyond) + P * gather (gather_lds for a filter in LDS) when the probe
+ * streams the next column (P > 1/4), scan + lines(P) * line + P * gather_queue
+ * (gather_queue_lds) when it queues the passing rows, blended for P in
+ * [0.2, 0.3] (and no gather
+ * term for an exact filter of a semi-join), with
  * P = fact_pass + (1 - fact_pass) * false-positive rate(bits, k, layout) and
  * lines(P) = 1 - (1 - P)^32 the fraction of 128-byte lines of the next column
  * that hold a passing row; the constants are the context's (defaults, or
@@ -412,9 +416,17 @@ lj_status lj_tune_calibrate(lj_ctx* ctx);
 typedef struct {
   double scan_l2;          /* scan with the first filter in L2 (up to 2 MiB), >= 1 */
   double scan_big;         /* scan with a larger filter, >= scan_l2 */
-  double probe_extra;      /* each further bit probe of a standard k > 1 filter, >= 0 */
+  double probe_extra;      /* each further bit probe of a standard k > 1 filter outside LDS,
+                              times its scan, >= 0 */
   double line;             /* a 128-byte line of the next column that holds a passing row, >= 0 */
-  double gather;           /* a passing row (queue, hash-slot gather), >= 0 */
+  double gather;           /* a passing row when more than a quarter pass (the measure
+                              streams with the keys), filter outside LDS, >= 0 */
+  double scan_mod;         /* scan with a modulo filter in LDS, > 0 */
+  double scan_blocked;     /* scan with a blocked multiplicative filter in LDS, > 0 */
+  double gather_queue;     /* a passing row when at most a quarter pass (hash-slot and
+                              measure gathers), >= 0 */
+  double gather_lds;       /* as gather, with the filter in LDS, >= 0 */
+  double gather_queue_lds; /* as gather_queue, with the filter in LDS, >= 0 */
   int32_t calibrated;      /* get: 1 once measured or set */
   int32_t reserved;
 } lj_tune_model;

@@ -56,6 +56,8 @@ def test_host_generator_argument_checks():
 
 def test_tune_model_argument_checks():
     lib = _abi.load()
-    m = _abi.TuneModel(scan_l2=2.0, scan_big=3.0, probe_extra=0.2, line=1.0, gather=6.0)
+    m = _abi.TuneModel(scan_l2=2.0, scan_big=3.0, probe_extra=0.2, line=1.0, gather=6.0, scan_mod=0.5,
+                       scan_blocked=0.7, gather_queue=8.0, gather_lds=9.0,
+                       gather_queue_lds=14.0)
     assert lib.lj_tune_set_model(None, C.byref(m)) == _abi.EINVAL
     assert lib.lj_tune_get_model(None, C.byref(m)) == _abi.EINVAL

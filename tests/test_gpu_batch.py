@@ -159,11 +159,13 @@ def test_tune_model_get_set():
         assert m["calibrated"] == 1 and m["scan_l2"] >= 1.0 and m["scan_big"] >= m["scan_l2"]
         b.set_tune_model(m)
         got = b.tune_model()
-        assert all(got[k] == m[k] for k in ("scan_l2", "scan_big", "probe_extra", "line", "gather"))
+        assert all(got[k] == m[k] for k in ("scan_l2", "scan_big", "probe_extra", "line", "gather", "scan_mod",
+                                            "scan_blocked", "gather_queue", "gather_lds",
+                                            "gather_queue_lds"))
         for args in [(200000, 1 << 22, 0.05, False), (1 << 21, 1 << 22, 0.3, True), (5000, 1 << 20, 0.01, False)]:
             assert a.tune_bloom(*args[:3], semi_join=args[3]) == b.tune_bloom(*args[:3], semi_join=args[3])
         for bad in [dict(m, scan_l2=0.5), dict(m, scan_big=m["scan_l2"] - 0.1), dict(m, gather=-1.0),
-                    dict(m, line=float("nan"))]:
+                    dict(m, line=float("nan")), dict(m, scan_mod=0.0), dict(m, gather_queue=-2.0)]:
             with pytest.raises(L.LJError):
                 b.set_tune_model(bad)
         assert b.tune_model() == got
