@@ -1020,9 +1020,11 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   // that fall back to global words: all_k0 is cleared below).  Such a probe
   // is bound by the rate of L2 requests of its filter reads, and the staged
   // share of them leaves that stream (2^28 fact rows over a 2^22-bit filter:
-  // 1.38 -> 0.99 ms).  A join that probes the table for each survivor is
-  // bound by those gathers instead, and the generic body makes it slower
-  // (2.40 -> 2.92 ms): it keeps the whole filter in global memory.  Only
+  // 1.38 -> 0.99 ms).  A join that probes the table for each survivor gains
+  // while few rows pass (10 %: 1.58 -> 1.41 ms) and loses once the gathers
+  // bound it (50 %: 2.42 -> 2.91 ms; the global reads past the prefix stall
+  // the streamed columns' loads), and its pass rate is not known when it is
+  // planned: it keeps the whole filter in global memory.  Only
   // when the fact rows repay each workgroup's staging copy kPrefixRepay
   // times over: the probes moved to LDS, n * staged / words, against the
   // copy, a 128-byte line per 32 staged words on each of about n_cu
