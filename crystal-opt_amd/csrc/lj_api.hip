@@ -1134,6 +1134,7 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   a.lds_filter_words = lds_words;
   a.goffset = q->group_offset;
   a.n_groups = q->n_groups;
+  a.gmod = make_fastmod((uint32_t)q->n_groups);
   for (int d = 0; d < nd; d++) a.dim_counters[d] = q->resident[d] ? nullptr : p.dims[d]->counters;
   return LJ_OK;
 }
@@ -2684,6 +2685,7 @@ static lj_status serve_warm(lj_server* S) {
   a.n = 0;
   a.n_dims = 1;
   a.n_groups = 1;
+  a.gmod = make_fastmod(1);
   a.gslot_bits = ilog2_u64((uint64_t)kGroupSlots);
   a.exact_eager_div = c->exact_eager_div;
   for (int i = 0; i < LJ_MAX_DIMS; i++) {

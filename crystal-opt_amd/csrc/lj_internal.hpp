@@ -110,6 +110,7 @@ struct ProbeArgs {
                            // stage 0's kind); 0 = any
   int64_t goffset;
   int64_t n_groups;
+  FastMod gmod;            // n_groups as a 32-bit divisor (n_groups <= 2^26)
   unsigned long long* gsum;
   unsigned long long* gcount;
   unsigned long long* gprod;

@@ -518,7 +518,9 @@ __device__ __forceinline__ int group_add(const ProbeArgs& a, GroupLds& gt, bool 
 #pragma unroll
     for (int t = 0; t < kMaxDims; t++)
       if (t < nd) acc += (int64_t)pay.v[t] * a.dim[t].gmult;
-    g = acc % a.n_groups;
+    // (a 64-bit remainder is a long instruction sequence; the group index
+    // almost always fits 32 bits)
+    g = acc >= 0 && acc <= (int64_t)UINT32_MAX ? (int64_t)fastmod_u32((uint32_t)acc, a.gmod) : acc % a.n_groups;
   }
   const int n_bad = __popcll(__ballot(alive && g < 0));
   if (!alive || g < 0) return n_bad;
