@@ -116,7 +116,7 @@ struct lj_ctx {
   double tune_gather = 5.7;      // a passing row when more than a quarter pass (the measure streams), L2 filter
   double tune_gather_lds = 9.3;  // the same with the filter in LDS (nothing else to overlap the gathers with)
   double tune_gather_q = 5.3;    // a passing row when fewer pass (queue, hash-slot and measure gathers), L2 filter
-  double tune_gather_q_lds = 15.0;  // the same with the filter in LDS (its short scan hides no gather)
+  double tune_gather_q_lds = 7.5;   // the same with the filter in LDS (its short scan hides no gather)
   int tune_calibrated = 0;
   std::map<std::pair<int, size_t>, int> occupancy;  // (config key, LDS bytes) -> workgroups per CU
   std::multimap<size_t, void*> pinned_free;         // pinned host blocks for reuse (batches)
@@ -2097,7 +2097,7 @@ constexpr int kCalBlocked = 100;
 // LDS that nothing passes (standard multiplicative, the unit; modulo;
 // blocked), a 2^22-bit (L2) and a 2^25-bit filter that nothing passes, the
 // exact 2^22-bit filter that 20 % and 40 % pass, and the 2^20-bit modulo
-// filter in LDS over 1 % and 10 % of the keys (4 keys a bit: about 5 % and
+// filter in LDS over 3 % and 10 % of the keys (4 keys a bit: about 14 % and
 // 41 % pass; survivor costs are measured against the empty filter of the
 // same kind).
 
@@ -2176,7 +2176,7 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
     *per_row = (t_all - t_quarter) / (0.75f * (float)nf);
     return st;
   };
-  float t_lds = 0, t_mod = 0, t_blk = 0, t_l2 = 0, t_big = 0, t_p20 = 0, t_p40 = 0, t_m5 = 0, t_m41 = 0;
+  float t_lds = 0, t_mod = 0, t_blk = 0, t_l2 = 0, t_big = 0, t_p20 = 0, t_p40 = 0, t_m14 = 0, t_m41 = 0;
   // the survivor costs are measured with every passing row reading the table
   // (these one-group joins would otherwise let the exact filter decide)
   const int exact_saved = c->exact_filter;
@@ -2185,7 +2185,7 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
       (s = run(kCalBlocked, 1 << 20, 0, &t_blk)) != LJ_OK || (s = run(LJ_BLOOM_MOD, nd, 0, &t_l2)) != LJ_OK ||
       (s = run(LJ_BLOOM_MOD, 1 << 25, 0, &t_big)) != LJ_OK || (s = run(LJ_BLOOM_MOD, nd, 2000, &t_p20)) != LJ_OK ||
       (s = run(LJ_BLOOM_MOD, nd, 4000, &t_p40)) != LJ_OK ||
-      (s = run(LJ_BLOOM_MOD, 1 << 20, 100, &t_m5)) != LJ_OK || (s = run(LJ_BLOOM_MOD, 1 << 20, 1000, &t_m41)) != LJ_OK) {
+      (s = run(LJ_BLOOM_MOD, 1 << 20, 300, &t_m14)) != LJ_OK || (s = run(LJ_BLOOM_MOD, 1 << 20, 1000, &t_m41)) != LJ_OK) {
     c->exact_filter = exact_saved;
     release();
     return s;
@@ -2204,7 +2204,9 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
   // (line stays the model's: the lines term barely changes between the pass
   // rates a short calibration can time, so it cannot be fitted from them;
   // 0.36 fits a 2^28-row sweep from 1 % to 19 %), and gather_q_lds from the
-  // LDS filter that passes about 5 %.  Above a quarter the measure streams
+  // LDS filter that passes about 14 % (the queued cost per row falls as more
+  // rows pass; the filters the tuner weighs against each other pass 5-20 %).
+  // Above a quarter the measure streams
   // with the keys: gather from 40 %, gather_lds from the LDS filter that
   // passes about 41 %.
   const double e20 = std::max(0.0, ((double)t_p20 - t_l2) / t_lds);
@@ -2213,11 +2215,11 @@ extern "C" lj_status lj_tune_calibrate(lj_ctx* c) {
   const auto p_mod20 = [&](double sel) {  // the 2^20-bit modulo filter over sel of the keys
     return sel + (1.0 - sel) * fpr_of(LJ_BLOOM_MOD, LJ_BLOOM_STANDARD, 1, sel * (double)nd, (double)(1 << 20), (double)nd);
   };
-  const double p5 = p_mod20(0.01), p41 = p_mod20(0.1);
-  const double e5 = std::max(0.0, ((double)t_m5 - t_mod) / t_lds);
+  const double p14 = p_mod20(0.03), p41 = p_mod20(0.1);
+  const double e14 = std::max(0.0, ((double)t_m14 - t_mod) / t_lds);
   const double e41 = std::max(0.0, ((double)t_m41 - t_mod) / t_lds);
   c->tune_gather_q = std::max(1.0, gather_q);
-  c->tune_gather_q_lds = std::max(1.0, (e5 - lines_of(p5) * c->tune_line) / p5);
+  c->tune_gather_q_lds = std::max(1.0, (e14 - lines_of(p14) * c->tune_line) / p14);
   c->tune_gather = std::max(1.0, e40 / 0.4);
   c->tune_gather_lds = std::max(1.0, e41 / p41);
   c->tune_calibrated = 1;
