@@ -103,6 +103,8 @@ struct lj_ctx {
   int filter_prefix = 1;         // LJ_FILTER_PREFIX=0: stage whole modulo filters
   int partial_prefix = 1;        // LJ_PARTIAL_PREFIX=0: never stage part of a filter that does not fit
   int exact_eager_div = 32;      // LJ_EXACT_EAGER: an exact semi-join streams its measure above 1/this pass
+  int direct1 = 1;               // LJ_DIRECT=0: one-dimension one-group joins queue every survivor
+  int direct_eager_div = 4;      // LJ_DIRECT_EAGER: ... and are summed in place above 1/this pass
   double clock_khz = 100000.0;  // device wall clock (kernel phase timestamps)
   // Bloom autotuner cost model, per fact row, relative to the scan of an LDS filter
   // cost model of the Bloom autotuner, per fact row, in units of a scan with
@@ -444,6 +446,8 @@ extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* 
   c->filter_prefix = env_int("LJ_FILTER_PREFIX", 1);
   c->partial_prefix = env_int("LJ_PARTIAL_PREFIX", 1);
   c->exact_eager_div = std::max(1, env_int("LJ_EXACT_EAGER", 32));
+  c->direct1 = env_int("LJ_DIRECT", 1);
+  c->direct_eager_div = std::max(1, env_int("LJ_DIRECT_EAGER", 4));
   // lj_star_join_noalloc's scratch on this device: its address, once per
   // device (the device is current), and the one clear of its zero region
   if (device < kNaMaxDevices) {
@@ -947,6 +951,14 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
         known_in)
       P.exact = 1;
   }
+  // A one-dimension, one-group LIP join over a perfect-hash table: where the
+  // filter passes enough rows for the measure to stream, the survivors probe
+  // the table and are summed in place instead of queued (a modulo filter:
+  // the bodies of the other kinds keep the queue).
+  if (lip && nd == 1 && q->n_groups == 1 && c->direct1) {
+    const int32_t k = p.dims[0]->desc.ht_kind;
+    a.direct1 = k == LJ_HT_PHT1 || k == LJ_HT_PHT2;
+  }
   // Whole filters, or their prefixes (filter_prefix_words): prefixes free
   // LDS for a larger group table when the group space is small enough for
   // every group to be hit (at most kRepGroups), and let more filters be
@@ -1115,6 +1127,7 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   a.want_product = want_product;
   a.k0 = cfg.k0;
   a.exact_eager_div = c->exact_eager_div;
+  a.direct_eager_div = c->direct_eager_div;
   for (int i = 0; i < LJ_MAX_DIMS; i++) {
     a.order[i] = i < nd ? q->order[i] : i;
     a.load_order[i] = i < nd ? (lip ? p.bloom_order[i] : q->order[i]) : i;
@@ -2690,6 +2703,7 @@ static lj_status serve_warm(lj_server* S) {
   a.gmod = make_fastmod(1);
   a.gslot_bits = ilog2_u64((uint64_t)kGroupSlots);
   a.exact_eager_div = c->exact_eager_div;
+  a.direct_eager_div = c->direct_eager_div;
   for (int i = 0; i < LJ_MAX_DIMS; i++) {
     a.order[i] = a.load_order[i] = i;
     a.dim[i].bloom_lds = -1;

@@ -137,6 +137,9 @@ struct ProbeArgs {
   int32_t k0;              // the query's stage-0 kind (S0Kind; read by probe_mixed_kernel)
   int32_t exact_eager_div;  // an exact one-dimension semi-join streams its measure with the keys when
                             // stage 0 passes more than 1 / exact_eager_div of its first tile's rows
+  int32_t direct1;         // one dimension, one group, a perfect-hash table: while the measure
+                           // streams, the survivors' table slots are gathered and summed in place
+  int32_t direct_eager_div;  // ... the measure streams above 1 / direct_eager_div pass
   // if set (a single query): the header and the first host_groups groups are
   // also written to this pinned host memory, so no copy follows the probe
   long long* host_hdr;

@@ -218,7 +218,7 @@ __device__ __forceinline__ bool exact_key(const ProbeDim& D, int32_t key) {
 // call site is thousands of instructions, and walking them misses the
 // instruction cache.  SPEC: 0 tests all_k0 here, 1 assumes it (the caller
 // tested it once for a whole loop), 2 assumes it is not set, 3 is the
-// one-dimension semi-join body (one filter: all_k0 holds).
+// one-dimension in-place body (one filter: all_k0 holds).
 template <int K0, int SPEC = 0>
 __device__ __forceinline__ bool probe_filter(const ProbeArgs& a, const ProbeDim& D, const uint32_t* lds,
                                              int32_t key) {
@@ -575,8 +575,10 @@ __device__ __forceinline__ int batch_lines(bool alive, int32_t row, int32_t& las
 struct WaveCtx {
   int nd;
   bool exact;               // one dimension whose exact filter replaces the table probe
-  long long acc_s;          // exact: the lane's SUM(measure) and row count, added in place
-  uint32_t acc_c;
+  bool direct;              // one dimension, one group: streamed survivors probe the table in place
+  long long acc_s;          // exact / direct: the lane's SUM(measure), row count and
+  uint32_t acc_c;           // SUM(measure * payload), added in place
+  long long acc_p;
   int ld0, ld1, ld2, ld3;   // column load order (stage -> dim)
   int32_t* q_row;
   int32_t* q_key;
@@ -1035,6 +1037,39 @@ __device__ __forceinline__ bool exact_in_place(const ProbeArgs& a, WaveCtx& w, W
   return true;
 }
 
+// One dimension, one group, a perfect-hash table (direct), the measure
+// streamed with the keys: a segment's survivors of the filter probe the table
+// here -- the slots of a lane's 4 rows gathered together (a dead row reads
+// slot 0 and drops it, so no branch separates the loads) -- and their
+// measures and products go to the lane's sums.  No queue: the same hits
+// (a nonzero slot), sums and statistics as the queued rows' complete_batch.
+__device__ __forceinline__ void direct_in_place(const ProbeArgs& a, WaveCtx& w, WaveStats& st, const v4i32& kv,
+                                                const v4i32& mv, uint32_t m) {
+  const ProbeDim& D = a.dim[0];
+  int64_t sl[4];
+  uint64_t raw[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    sl[j] = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, vget(kv, j));
+    raw[j] = ht_load(D.ht_kind, D.ht, ((m >> j) & 1u) && sl[j] >= 0 ? sl[j] : 0);
+  }
+  uint32_t hit = 0;
+  long long s = 0, p = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const bool h = ((m >> j) & 1u) && sl[j] >= 0 && raw[j] != 0;
+    const int32_t mj = vget(mv, j);
+    hit |= (uint32_t)h << j;
+    s += h ? (long long)mj : 0ll;
+    p += h ? (long long)(int32_t)((uint32_t)mj * (uint32_t)(raw[j] >> 32)) : 0ll;
+  }
+  st.meas_rows += popc_ballot4(hit);
+  st.meas_lines += lines_of_ballot(__ballot(hit != 0));
+  w.acc_s += s;
+  w.acc_c += (uint32_t)__popc(hit);
+  if (a.want_product) w.acc_p += p;
+}
+
 struct TileCtx {
   uint32_t n;        // fact rows
   uint32_t start;    // first row of the tile space (after the adaptive sample)
@@ -1111,6 +1146,11 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
     }
     if (SPEC == 3 && have_eager && w.exact && exact_in_place(a, w, st, kv, ev, m)) {
       st.v[ST_BLOOM_PASS] += cnt;  // every survivor passed the only filter
+      continue;
+    }
+    if (SPEC == 3 && have_eager && w.direct) {
+      direct_in_place(a, w, st, kv, ev, m);
+      st.v[ST_BLOOM_PASS] += cnt;
       continue;
     }
     if (cnt > kDenseMin || have_eager) {
@@ -1453,6 +1493,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   w.ld3 = a.load_order[3];
   w.acc_s = 0;
   w.acc_c = 0;
+  w.acc_p = 0;
   w.exact = SPEC == 3 && a.n_dims == 1 && a.dim[0].exact != 0;
   if (w.exact && a.dim_counters[0]) {
     // a dimension built with the query: a selected key outside the table's
@@ -1462,6 +1503,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
     for (int l = 0; l < kCounterLanes; l++) o += a.dim_counters[0][2 * l + 1];
     w.exact = o == 0;
   }
+  w.direct = SPEC == 3 && !w.exact && a.n_dims == 1 && a.n_groups == 1 && a.direct1 != 0;
 
   const uint32_t total_waves = nblk * NW;
   const uint32_t gw = bid * NW + wave;
@@ -1535,7 +1577,9 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       // (the exact semi-join finishes a streamed segment in place, so it
       // streams from a lower pass rate than the queue's gathers would pay)
       if (pass == 0 && !a.adaptive)
-        eager = (uint32_t)tot * (SPEC == 3 && w.exact ? (uint32_t)a.exact_eager_div : 4u) > nv;
+        eager = (uint32_t)tot * (SPEC == 3 && w.exact    ? (uint32_t)a.exact_eager_div
+                                 : SPEC == 3 && w.direct ? (uint32_t)a.direct_eager_div
+                                                         : 4u) > nv;
     }
     if (pass == 0) {
       TileSched<NW> sch;
@@ -1551,12 +1595,14 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
   for (int it = 0; it < 2; it++) rotate_batch<K0, SPEC>(a, lds_filter, gt, w, st);
 
   // ---- flush: statistics and groups ----------------------------------------------
-  if (SPEC == 3 && w.exact) {
+  if (SPEC == 3 && (w.exact || w.direct)) {
     const long long es = WaveSum(w.acc_s);
     const unsigned long long ec = WaveSum((unsigned long long)w.acc_c);
+    const long long ep = a.want_product ? WaveSum(w.acc_p) : 0ll;
     if (lane == 0 && ec != 0) {
       atomicAdd(&gt.sum[0], (unsigned long long)es);
       atomicAdd(&gt.cnt[0], ec);
+      if (a.want_product) atomicAdd(&gt.prod[0], (unsigned long long)ep);
       if (gt.key[0] == kEmptyGroup) {
         gt.key[0] = 0;
         for (int d = 0; d < kMaxDims; d++) gt.pay[d] = 0;
@@ -1629,9 +1675,10 @@ __device__ __forceinline__ void probe_dispatch(const ProbeArgs& a, uint32_t bid,
   }
   constexpr bool is_mod = K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_GL;
   if constexpr (is_mod) {
-    // a one-dimension semi-join through an exact filter: a body of its own
-    // (its in-place sums cost registers the other bodies do not pay for)
-    if (a.n_dims == 1 && a.dim[0].exact) {
+    // one dimension summed in place (a semi-join through an exact filter, or
+    // a one-group join over a perfect-hash table): a body of its own (its
+    // in-place sums cost registers the other bodies do not pay for)
+    if (a.n_dims == 1 && (a.dim[0].exact || a.direct1)) {
       probe_body<WG, VEC, K0, 3>(a, bid, nblk);
       return;
     }

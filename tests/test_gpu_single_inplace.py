@@ -1,0 +1,111 @@
+"""One-dimension, one-group LIP joins summed in place.
+
+When the Bloom filter passes enough rows for the measure column to stream
+with the keys, a single join over a perfect-hash table (PHT_1 or PHT_2)
+probes the table and adds each survivor's measure and product in place
+instead of queueing it (LJ_DIRECT, on by default).  Every result must equal
+the queued path's (a context created with LJ_DIRECT=0) and the oracle's:
+match count, SUM(measure), SUM(measure * payload) (where no two selected
+keys share a table slot), filter pass count and,
+for the static plan, the per-stage row and line counts.  The cases cover
+both table kinds, a table shorter than the key range (slots alias, as the
+reference's modulo hash does), pass rates on either side of the streaming
+threshold, a lowered threshold (LJ_DIRECT_EAGER), the adaptive plan, and
+ragged and tiny fact tables.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import lipjoin as L
+import oracle as O
+from lipjoin import workloads as W
+from q33_helpers import to_device
+
+pytestmark = pytest.mark.gpu
+
+SEED = W.seed_of(7)
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _context_with(env):
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return L.Context(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.fixture(scope="module")
+def ctx_queue():
+    c = _context_with({"LJ_DIRECT": "0"})
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def ctx_low_threshold():
+    c = _context_with({"LJ_DIRECT_EAGER": "32"})
+    yield c
+    c.close()
+
+
+# (n_dim, n_fact, ht_len divisor, threshold on attr (of 0..99), filter bits, table kind)
+CASES = [
+    (1 << 16, (1 << 20) + 13, 1, 50, 1 << 14, L.HT_PHT2),   # passes ~85 %: streamed, in place
+    (1 << 16, (1 << 20) + 13, 1, 50, 1 << 14, L.HT_PHT1),   # keys-only table: products are 0
+    (1 << 16, (1 << 20) + 7, 3, 40, 1 << 15, L.HT_PHT2),    # table of n_dim/3 slots: keys alias
+    (1 << 16, (1 << 20) + 1, 1, 5, 1 << 20, L.HT_PHT2),     # passes ~5 %: the queue
+    (1 << 16, (1 << 20) + 3, 1, 12, 1 << 17, L.HT_PHT2),    # passes ~12 %: queue, or in place when lowered
+    (1 << 12, 4099, 1, 70, 1 << 10, L.HT_PHT2),              # tiny, ragged
+    (1 << 12, 100, 1, 70, 1 << 10, L.HT_PHT2),               # fewer rows than one tile
+]
+
+
+def _run(c, dev, n_dim, n_fact, ht_len, thr, bits, kind, method):
+    desc = L.dim_desc(dev["keys"], n_dim, ht_len, attr=dev["attr"], predicate=W.single_join_pred(thr),
+                      bloom=("mod", bits), ht_kind=kind)
+    return c.hash_join(dev["fk"], dev["meas"], n_fact, desc, method=method)
+
+
+def _key(r, product=True):
+    return (r["n_match"], r["sum_measure"], r["sum_product"] if product else None, r["n_bloom_pass"])
+
+
+@pytest.mark.parametrize("method", ["lip", "lip-adaptive"])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_in_place_equals_queue_and_oracle(ctx, ctx_queue, ctx_low_threshold, case, method):
+    n_dim, n_fact, div, thr, bits, kind = CASES[case]
+    ht_len = n_dim // div
+    _flush(f"in-place single join case {case} {method}: n_dim {n_dim} n_fact {n_fact} ht_len {ht_len} "
+           f"thr {thr} bits {bits} kind {kind}")
+    keys, attr, fk, meas = W.single_join_host(SEED + case, n_dim, n_fact, 99)
+    dev = to_device({"keys": keys, "attr": attr, "fk": fk, "meas": meas})
+    ref = O.single_join(O.DimSpec(keys, attr, ht_len, pred=list(W.single_join_pred(thr)), ht_kind=kind,
+                                  bloom=O.bloom_spec(O.BLOOM_MOD, bits, 1)), fk, meas, method=O.LIP, threads=8)
+    # two selected keys in one slot (an aliased table): the slot keeps the
+    # payload of whichever row the build wrote first, so the product is not
+    # deterministic -- the reference's CAS build has the same race
+    product = div == 1
+    exp = (ref["n_match"], ref["sum_measure"], ref["sum_product"] if product else None, ref["n_bloom_pass"])
+    results = [_run(c, dev, n_dim, n_fact, ht_len, thr, bits, kind, method)
+               for c in (ctx, ctx_queue, ctx_low_threshold)]
+    torch.cuda.synchronize()
+    for i, r in enumerate(results):
+        assert _key(r, product) == exp, (i, _key(r, product), exp)
+        if kind == L.HT_PHT1:
+            assert r["sum_product"] == 0
+        if method == "lip":
+            st = r["stats"]
+            assert st["rows_in"][:2] == [n_fact, ref["n_match"]], i
+            assert st["lines_in"][:2] == [ref["lines_fk"], ref["lines_measure"]], i
