@@ -104,7 +104,10 @@ struct lj_ctx {
   int partial_prefix = 1;        // LJ_PARTIAL_PREFIX=0: never stage part of a filter that does not fit
   int exact_eager_div = 32;      // LJ_EXACT_EAGER: an exact semi-join streams its measure above 1/this pass
   int direct1 = 1;               // LJ_DIRECT=0: one-dimension one-group joins queue every survivor
+                                 // (2: the in-place body whatever the filter's estimated pass rate)
   int direct_eager_div = 4;      // LJ_DIRECT_EAGER: ... and are summed in place above 1/this pass
+  int direct_occ = 1;            // LJ_DIRECT_OCC: the table's occupancy bitmap in LDS for them
+                                 // (0 never, 1 when the fact rows repay it, 2 whenever it fits)
   double clock_khz = 100000.0;  // device wall clock (kernel phase timestamps)
   // Bloom autotuner cost model, per fact row, relative to the scan of an LDS filter
   // cost model of the Bloom autotuner, per fact row, in units of a scan with
@@ -448,6 +451,7 @@ extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* 
   c->exact_eager_div = std::max(1, env_int("LJ_EXACT_EAGER", 32));
   c->direct1 = env_int("LJ_DIRECT", 1);
   c->direct_eager_div = std::max(1, env_int("LJ_DIRECT_EAGER", 4));
+  c->direct_occ = env_int("LJ_DIRECT_OCC", 1);
   // lj_star_join_noalloc's scratch on this device: its address, once per
   // device (the device is current), and the one clear of its zero region
   if (device < kNaMaxDevices) {
@@ -954,10 +958,25 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   // A one-dimension, one-group LIP join over a perfect-hash table: where the
   // filter passes enough rows for the measure to stream, the survivors probe
   // the table and are summed in place instead of queued (a modulo filter:
-  // the bodies of the other kinds keep the queue).
+  // the bodies of the other kinds keep the queue).  Its body pays from about
+  // a fifth of the rows passing: a modulo filter holding S selected keys
+  // passes about 1 - exp(-S / min(bits, ht_len)) of uniform fact keys, so
+  // S >= -ln(4/5) min(bits, ht_len).  Known here for a resident dimension;
+  // the probe checks it on the build's counters for one built with the query.
   if (lip && nd == 1 && q->n_groups == 1 && c->direct1) {
-    const int32_t k = p.dims[0]->desc.ht_kind;
-    a.direct1 = k == LJ_HT_PHT1 || k == LJ_HT_PHT2;
+    const lj_dim* D = p.dims[0];
+    const int32_t k = D->desc.ht_kind;
+    const ProbeDim& P = a.dim[0];
+    if ((k == LJ_HT_PHT1 || k == LJ_HT_PHT2) && (P.bh.kind == BLOOM_MOD || P.bh.kind == BLOOM_MOD_POW2)) {
+      const double dom = (double)std::min<uint32_t>(P.bh.mod.d, P.ht_len.d);
+      const int64_t min_sel = c->direct1 >= 2 ? 0 : (int64_t)std::ceil(0.2231 * dom);
+      if (q->resident[0]) {
+        a.direct1 = D->n_selected >= min_sel;
+      } else {
+        a.direct1 = 1;
+        a.direct_min_sel = (int32_t)min_sel;
+      }
+    }
   }
   // Whole filters, or their prefixes (filter_prefix_words): prefixes free
   // LDS for a larger group table when the group space is small enough for
@@ -1101,6 +1120,24 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   for (int d = 0; d < nd; d++)
     if (lip && family(kind_of(d)) != family(cfg.k0)) a.all_k0 = 0;
   if (partial) a.all_k0 = 0;
+  // A table probed in place (direct1, behind a modulo filter: the in-place
+  // body) whose occupancy bitmap fits in the LDS left: each workgroup builds
+  // the bitmap from the table at its start (ht_len slots read from L2), and
+  // the probe then decides hit or miss with an LDS read and gathers only the
+  // hits' payloads.  Repaid when each CU's share of the fact rows is at least
+  // the table's slot count (about 8 bytes read per slot at the start, against
+  // a slot gather saved per passing row).
+  if (a.direct1 && !partial && c->direct_occ > 0 &&
+      (a.dim[0].bh.kind == BLOOM_MOD || a.dim[0].bh.kind == BLOOM_MOD_POW2)) {
+    const int64_t ns = a.dim[0].ht_len.d;
+    const int64_t ow = (((ns + 63) / 64) * 2 + 3) & ~3ll;
+    const bool repaid = c->direct_occ >= 2 || (double)q->n_fact >= (double)c->n_cu * (double)ns;
+    if (repaid && lds_words + ow <= lds_budget_words) {
+      a.occ_lds = lds_words;
+      a.occ_words = (int32_t)ow;
+      lds_words += (int)ow;
+    }
+  }
   // the LDS group table: 32 slots, or for a small group space up to twice
   // its size (at most kGroupSlotsMax) in the LDS that staging leaves.  A
   // 1024-thread workgroup is alone on its CU (the probe's registers allow 16

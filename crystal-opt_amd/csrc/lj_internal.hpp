@@ -140,6 +140,10 @@ struct ProbeArgs {
   int32_t direct1;         // one dimension, one group, a perfect-hash table: while the measure
                            // streams, the survivors' table slots are gathered and summed in place
   int32_t direct_eager_div;  // ... the measure streams above 1 / direct_eager_div pass
+  int32_t occ_lds;         // direct1 with occ_words > 0: the table's occupancy bitmap (bit s set iff
+  int32_t occ_words;       // slot s is nonzero), built in LDS at this word offset by each workgroup
+  int32_t direct_min_sel;  // direct1, a dimension built with the query: the in-place body runs when
+                           // its build selected at least this many keys (0: always)
   // if set (a single query): the header and the first host_groups groups are
   // also written to this pinned host memory, so no copy follows the probe
   long long* host_hdr;

@@ -45,7 +45,8 @@
 
 namespace lj {
 
-constexpr int kStageUnroll = 8;     // 16-byte filter loads in flight per thread while staging
+constexpr int kStageUnroll = 8;
+constexpr int kOccUnroll = 8;   // table loads in flight per lane while the occupancy bitmap builds     // 16-byte filter loads in flight per thread while staging
 constexpr int kFinUnroll = 8;       // statistics rows per thread read together by the finalize
 constexpr int kDenseMin = 8;         // stage-0 survivors of a segment above which it is processed in place
 constexpr int kGroupProbes = 32;     // slots a row tries in the LDS group table before going global
@@ -1043,23 +1044,43 @@ __device__ __forceinline__ bool exact_in_place(const ProbeArgs& a, WaveCtx& w, W
 // slot 0 and drops it, so no branch separates the loads) -- and their
 // measures and products go to the lane's sums.  No queue: the same hits
 // (a nonzero slot), sums and statistics as the queued rows' complete_batch.
-__device__ __forceinline__ void direct_in_place(const ProbeArgs& a, WaveCtx& w, WaveStats& st, const v4i32& kv,
-                                                const v4i32& mv, uint32_t m) {
+// With the table's occupancy bitmap in LDS (occ_words), the bit decides the
+// hit and only the hits' slots are gathered, for their payloads (products
+// over a PHT_2 table); the others read slot 0, one shared line.
+__device__ __forceinline__ void direct_in_place(const ProbeArgs& a, const uint32_t* lds, WaveCtx& w, WaveStats& st,
+                                                const v4i32& kv, const v4i32& mv, uint32_t m) {
   const ProbeDim& D = a.dim[0];
   int64_t sl[4];
   uint64_t raw[4];
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    sl[j] = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, vget(kv, j));
-    raw[j] = ht_load(D.ht_kind, D.ht, ((m >> j) & 1u) && sl[j] >= 0 ? sl[j] : 0);
-  }
   uint32_t hit = 0;
+  if (a.occ_words > 0) {
+    const uint32_t* occ = lds + a.occ_lds;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      sl[j] = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, vget(kv, j));
+      const bool on = ((m >> j) & 1u) && sl[j] >= 0;
+      const uint32_t x = on ? (uint32_t)sl[j] : 0u;
+      hit |= (uint32_t)(on && ((occ[x >> 5] >> (x & 31)) & 1u)) << j;
+      raw[j] = 0;
+    }
+    if (a.want_product && D.ht_kind == HT_PHT2 && __ballot(hit != 0)) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) raw[j] = ht_load(D.ht_kind, D.ht, ((hit >> j) & 1u) ? sl[j] : 0);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      sl[j] = ht_first_slot(D.ht_kind, D.ht_len, D.keys_min, vget(kv, j));
+      raw[j] = ht_load(D.ht_kind, D.ht, ((m >> j) & 1u) && sl[j] >= 0 ? sl[j] : 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) hit |= (uint32_t)(((m >> j) & 1u) && sl[j] >= 0 && raw[j] != 0) << j;
+  }
   long long s = 0, p = 0;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const bool h = ((m >> j) & 1u) && sl[j] >= 0 && raw[j] != 0;
+    const bool h = (hit >> j) & 1u;
     const int32_t mj = vget(mv, j);
-    hit |= (uint32_t)h << j;
     s += h ? (long long)mj : 0ll;
     p += h ? (long long)(int32_t)((uint32_t)mj * (uint32_t)(raw[j] >> 32)) : 0ll;
   }
@@ -1068,6 +1089,32 @@ __device__ __forceinline__ void direct_in_place(const ProbeArgs& a, WaveCtx& w, 
   w.acc_s += s;
   w.acc_c += (uint32_t)__popc(hit);
   if (a.want_product) w.acc_p += p;
+}
+
+// The occupancy bitmap of a table of ns slots (T: the slot type): a wave
+// takes 64 slots at a time, lane i slot 64c + i, and the ballot of the
+// nonzero ones is words 2c and 2c+1.  kOccUnroll chunks' loads go out
+// before their ballots.  Slot indexes past the end are clamped (the loads
+// stay unconditional) and their bits cleared.
+template <typename T, int NW>
+__device__ __forceinline__ void occupancy_build(const T* ht, uint32_t ns, uint32_t* occ, int wave) {
+  const uint32_t lane = lane_id();
+  const uint32_t nch = (ns + 63) / 64;
+  for (uint32_t c0 = (uint32_t)wave; c0 < nch; c0 += NW * kOccUnroll) {
+    T v[kOccUnroll];
+#pragma unroll
+    for (int u = 0; u < kOccUnroll; u++) {
+      const uint32_t s = (c0 + u * NW) * 64 + lane;
+      v[u] = gl(ht)[s < ns ? s : ns - 1];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < kOccUnroll; u++) {
+      const uint32_t c = c0 + u * NW;
+      const unsigned long long b = __ballot(c * 64 + lane < ns && v[u] != 0);
+      if (c < nch && lane < 2) occ[2 * c + lane] = lane ? (uint32_t)(b >> 32) : (uint32_t)b;
+    }
+  }
 }
 
 struct TileCtx {
@@ -1149,7 +1196,7 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
       continue;
     }
     if (SPEC == 3 && have_eager && w.direct) {
-      direct_in_place(a, w, st, kv, ev, m);
+      direct_in_place(a, lds, w, st, kv, ev, m);
       st.v[ST_BLOOM_PASS] += cnt;
       continue;
     }
@@ -1462,6 +1509,13 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       }
     }
   }
+  if (SPEC == 3 && a.occ_words > 0) {
+    const ProbeDim& D = a.dim[0];
+    if (D.ht_kind == HT_PHT1)
+      occupancy_build<uint32_t, NW>(reinterpret_cast<const uint32_t*>(D.ht), D.ht_len.d, lds_filter + a.occ_lds, wave);
+    else
+      occupancy_build<uint64_t, NW>(reinterpret_cast<const uint64_t*>(D.ht), D.ht_len.d, lds_filter + a.occ_lds, wave);
+  }
   for (int i = threadIdx.x; i < gslots; i += WG) {
     gt.key[i] = kEmptyGroup;
     gt.cnt[i] = 0;
@@ -1676,9 +1730,17 @@ __device__ __forceinline__ void probe_dispatch(const ProbeArgs& a, uint32_t bid,
   constexpr bool is_mod = K0 == S0_MODP2_LDS || K0 == S0_MOD_LDS || K0 == S0_MODP2_GL || K0 == S0_MOD_GL;
   if constexpr (is_mod) {
     // one dimension summed in place (a semi-join through an exact filter, or
-    // a one-group join over a perfect-hash table): a body of its own (its
-    // in-place sums cost registers the other bodies do not pay for)
-    if (a.n_dims == 1 && (a.dim[0].exact || a.direct1)) {
+    // a one-group join over a perfect-hash table whose filter passes enough
+    // rows): a body of its own (its in-place sums cost registers the other
+    // bodies do not pay for)
+    bool direct = a.direct1 != 0;
+    if (direct && a.direct_min_sel > 0 && a.dim_counters[0]) {
+      // a dimension built with the query: its selected keys, from the build's counters
+      unsigned long long sel = 0;
+      for (int l = 0; l < kCounterLanes; l++) sel += a.dim_counters[0][2 * l];
+      direct = sel >= (unsigned long long)a.direct_min_sel;
+    }
+    if (a.n_dims == 1 && (a.dim[0].exact || direct)) {
       probe_body<WG, VEC, K0, 3>(a, bid, nblk);
       return;
     }

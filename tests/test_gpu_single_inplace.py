@@ -10,8 +10,12 @@ keys share a table slot), filter pass count and,
 for the static plan, the per-stage row and line counts.  The cases cover
 both table kinds, a table shorter than the key range (slots alias, as the
 reference's modulo hash does), pass rates on either side of the streaming
-threshold, a lowered threshold (LJ_DIRECT_EAGER), the adaptive plan, and
-ragged and tiny fact tables.
+threshold, a lowered threshold (LJ_DIRECT_EAGER, with LJ_DIRECT=2: the
+in-place body even where the filter's selected keys predict few rows
+passing), the table's occupancy
+bitmap in LDS deciding the hits (LJ_DIRECT_OCC=2; a slot count that is not a
+multiple of 64 in the aliased case), the adaptive plan, and ragged and tiny
+fact tables.
 """
 import os
 
@@ -55,7 +59,25 @@ def ctx_queue():
 
 @pytest.fixture(scope="module")
 def ctx_low_threshold():
-    c = _context_with({"LJ_DIRECT_EAGER": "32"})
+    # the in-place body whatever the filter's estimated pass rate, streaming
+    # from 1/32 passing
+    c = _context_with({"LJ_DIRECT": "2", "LJ_DIRECT_EAGER": "32"})
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def ctx_occ():
+    # the table's occupancy bitmap in LDS whenever it fits (the default needs
+    # about CUs x table slots fact rows)
+    c = _context_with({"LJ_DIRECT_OCC": "2"})
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def ctx_occ_low():
+    c = _context_with({"LJ_DIRECT": "2", "LJ_DIRECT_OCC": "2", "LJ_DIRECT_EAGER": "32"})
     yield c
     c.close()
 
@@ -84,7 +106,7 @@ def _key(r, product=True):
 
 @pytest.mark.parametrize("method", ["lip", "lip-adaptive"])
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_in_place_equals_queue_and_oracle(ctx, ctx_queue, ctx_low_threshold, case, method):
+def test_in_place_equals_queue_and_oracle(ctx, ctx_queue, ctx_low_threshold, ctx_occ, ctx_occ_low, case, method):
     n_dim, n_fact, div, thr, bits, kind = CASES[case]
     ht_len = n_dim // div
     _flush(f"in-place single join case {case} {method}: n_dim {n_dim} n_fact {n_fact} ht_len {ht_len} "
@@ -99,7 +121,7 @@ def test_in_place_equals_queue_and_oracle(ctx, ctx_queue, ctx_low_threshold, cas
     product = div == 1
     exp = (ref["n_match"], ref["sum_measure"], ref["sum_product"] if product else None, ref["n_bloom_pass"])
     results = [_run(c, dev, n_dim, n_fact, ht_len, thr, bits, kind, method)
-               for c in (ctx, ctx_queue, ctx_low_threshold)]
+               for c in (ctx, ctx_queue, ctx_low_threshold, ctx_occ, ctx_occ_low)]
     torch.cuda.synchronize()
     for i, r in enumerate(results):
         assert _key(r, product) == exp, (i, _key(r, product), exp)
@@ -109,3 +131,32 @@ def test_in_place_equals_queue_and_oracle(ctx, ctx_queue, ctx_low_threshold, cas
             st = r["stats"]
             assert st["rows_in"][:2] == [n_fact, ref["n_match"]], i
             assert st["lines_in"][:2] == [ref["lines_fk"], ref["lines_measure"]], i
+
+
+@pytest.mark.parametrize("thr", [5, 50])
+def test_resident_dimension_matches_inline(ctx, thr):
+    """The in-place body is chosen from the filter's selected keys: on the
+    host for a resident dimension, on the device (the build's counters) for
+    one built with the query.  At 5 % selected (a 2^14-bit filter over
+    2^16 keys: about 18 % of the fact rows pass) neither chooses it, at 50 %
+    both do; either way the one-group star join equals the oracle's."""
+    n_dim, n_fact = 1 << 16, (1 << 20) + 5
+    _flush(f"resident vs inline one-dimension star join, threshold {thr}")
+    keys, attr, fk, meas = W.single_join_host(SEED + 20 + thr, n_dim, n_fact, 99)
+    dev = to_device({"keys": keys, "attr": attr, "fk": fk, "meas": meas})
+    pred = W.single_join_pred(thr)
+    desc = L.dim_desc(dev["keys"], n_dim, n_dim, attr=dev["attr"], predicate=pred, bloom=("mod", 1 << 14))
+    dim = ctx.build_dim(desc)
+    try:
+        a = ctx.star_join(L.star_desc([dev["fk"]], dev["meas"], n_fact, None, [0], resident=[dim]))
+        b = ctx.star_join(L.star_desc([dev["fk"]], dev["meas"], n_fact, [desc], [0]))
+    finally:
+        dim.release()
+    ref = O.single_join(O.DimSpec(keys, attr, n_dim, pred=list(pred), bloom=O.bloom_spec(O.BLOOM_MOD, 1 << 14, 1)),
+                        fk, meas, method=O.LIP, threads=8)
+    exp = (ref["n_match"], ref["sum_measure"])
+    assert (a["n_rows"], a["sum_total"]) == exp and (b["n_rows"], b["sum_total"]) == exp
+    for r in (a, b):
+        assert r["stats"]["rows_in"][:2] == [n_fact, ref["n_match"]]
+        assert r["stats"]["lines_in"][:2] == [ref["lines_fk"], ref["lines_measure"]]
+        assert r["stats"]["rows_bloom_pass"] == ref["n_bloom_pass"]
