@@ -108,6 +108,7 @@ struct lj_ctx {
   int direct_eager_div = 4;      // LJ_DIRECT_EAGER: ... and are summed in place above 1/this pass
   int direct_occ = 1;            // LJ_DIRECT_OCC: the table's occupancy bitmap in LDS for them
                                  // (0 never, 1 when the fact rows repay it, 2 whenever it fits)
+  int occ_eager_div = 8;         // LJ_DIRECT_OCC_EAGER: with it, the measure streams above 1/this hitting
   double clock_khz = 100000.0;  // device wall clock (kernel phase timestamps)
   // Bloom autotuner cost model, per fact row, relative to the scan of an LDS filter
   // cost model of the Bloom autotuner, per fact row, in units of a scan with
@@ -452,6 +453,7 @@ extern "C" lj_status lj_create(int32_t device, uint64_t hbm_budget_bytes, void* 
   c->direct1 = env_int("LJ_DIRECT", 1);
   c->direct_eager_div = std::max(1, env_int("LJ_DIRECT_EAGER", 4));
   c->direct_occ = env_int("LJ_DIRECT_OCC", 1);
+  c->occ_eager_div = std::max(1, env_int("LJ_DIRECT_OCC_EAGER", 8));
   // lj_star_join_noalloc's scratch on this device: its address, once per
   // device (the device is current), and the one clear of its zero region
   if (device < kNaMaxDevices) {
@@ -1165,6 +1167,7 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   a.k0 = cfg.k0;
   a.exact_eager_div = c->exact_eager_div;
   a.direct_eager_div = c->direct_eager_div;
+  a.occ_eager_div = c->occ_eager_div;
   for (int i = 0; i < LJ_MAX_DIMS; i++) {
     a.order[i] = i < nd ? q->order[i] : i;
     a.load_order[i] = i < nd ? (lip ? p.bloom_order[i] : q->order[i]) : i;
@@ -2741,6 +2744,7 @@ static lj_status serve_warm(lj_server* S) {
   a.gslot_bits = ilog2_u64((uint64_t)kGroupSlots);
   a.exact_eager_div = c->exact_eager_div;
   a.direct_eager_div = c->direct_eager_div;
+  a.occ_eager_div = c->occ_eager_div;
   for (int i = 0; i < LJ_MAX_DIMS; i++) {
     a.order[i] = a.load_order[i] = i;
     a.dim[i].bloom_lds = -1;

@@ -144,6 +144,8 @@ struct ProbeArgs {
   int32_t occ_words;       // slot s is nonzero), built in LDS at this word offset by each workgroup
   int32_t direct_min_sel;  // direct1, a dimension built with the query: the in-place body runs when
                            // its build selected at least this many keys (0: always)
+  int32_t occ_eager_div;   // with the bitmap: the measure streams above 1 / occ_eager_div of the
+                           // first tile's rows hitting (below, the hits' measures are gathered)
   // if set (a single query): the header and the first host_groups groups are
   // also written to this pinned host memory, so no copy follows the probe
   long long* host_hdr;

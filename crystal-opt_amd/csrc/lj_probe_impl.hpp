@@ -1046,9 +1046,13 @@ __device__ __forceinline__ bool exact_in_place(const ProbeArgs& a, WaveCtx& w, W
 // (a nonzero slot), sums and statistics as the queued rows' complete_batch.
 // With the table's occupancy bitmap in LDS (occ_words), the bit decides the
 // hit and only the hits' slots are gathered, for their payloads (products
-// over a PHT_2 table); the others read slot 0, one shared line.
-__device__ __forceinline__ void direct_in_place(const ProbeArgs& a, const uint32_t* lds, WaveCtx& w, WaveStats& st,
-                                                const v4i32& kv, const v4i32& mv, uint32_t m) {
+// over a PHT_2 table); the others read slot 0, one shared line.  Then the
+// measure need not stream: without it (have_meas false) the hits' measures
+// are gathered the same way (row 0 for the others).  Returns the segment's
+// hits (wave total).
+__device__ __forceinline__ int direct_in_place(const ProbeArgs& a, const uint32_t* lds, WaveCtx& w, WaveStats& st,
+                                               const v4i32& kv, const v4i32& mv, uint32_t m, uint32_t row0,
+                                               bool have_meas) {
   const ProbeDim& D = a.dim[0];
   int64_t sl[4];
   uint64_t raw[4];
@@ -1076,11 +1080,16 @@ __device__ __forceinline__ void direct_in_place(const ProbeArgs& a, const uint32
 #pragma unroll
     for (int j = 0; j < 4; j++) hit |= (uint32_t)(((m >> j) & 1u) && sl[j] >= 0 && raw[j] != 0) << j;
   }
+  int32_t mg[4] = {0, 0, 0, 0};
+  if (!have_meas && __ballot(hit != 0)) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) mg[j] = gl(a.measure)[((hit >> j) & 1u) ? row0 + j : 0u];
+  }
   long long s = 0, p = 0;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const bool h = (hit >> j) & 1u;
-    const int32_t mj = vget(mv, j);
+    const int32_t mj = have_meas ? vget(mv, j) : mg[j];
     s += h ? (long long)mj : 0ll;
     p += h ? (long long)(int32_t)((uint32_t)mj * (uint32_t)(raw[j] >> 32)) : 0ll;
   }
@@ -1089,6 +1098,7 @@ __device__ __forceinline__ void direct_in_place(const ProbeArgs& a, const uint32
   w.acc_s += s;
   w.acc_c += (uint32_t)__popc(hit);
   if (a.want_product) w.acc_p += p;
+  return popc_ballot4(hit);
 }
 
 // The occupancy bitmap of a table of ns slots (T: the slot type): a wave
@@ -1126,7 +1136,8 @@ struct TileCtx {
 
 // One tile: stage 0 on the registers `k` (VEC segments of 256 rows), then its
 // survivors.  CHECKED masks rows at or beyond n.  `ke` holds the next column
-// of the same rows when have_eager.  Returns the stage-0 survivor count.
+// of the same rows when have_eager.  Returns the stage-0 survivor count (the
+// table hits, for segments probed in place through the occupancy bitmap).
 template <int VEC, int K0, bool CHECKED, int SPEC>
 __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const uint32_t* lds, GroupLds& gt,
                                          WaveCtx& w, WaveStats& st, const TileCtx& tc, uint32_t base,
@@ -1195,9 +1206,10 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
       st.v[ST_BLOOM_PASS] += cnt;  // every survivor passed the only filter
       continue;
     }
-    if (SPEC == 3 && have_eager && w.direct) {
-      direct_in_place(a, lds, w, st, kv, ev, m);
+    if (SPEC == 3 && w.direct && (have_eager || a.occ_words > 0)) {
+      const int hits = direct_in_place(a, lds, w, st, kv, ev, m, row0, have_eager);
       st.v[ST_BLOOM_PASS] += cnt;
+      if (a.occ_words > 0) total += hits - cnt;  // with the bitmap, the measure streams by hits
       continue;
     }
     if (cnt > kDenseMin || have_eager) {
@@ -1632,7 +1644,8 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       // streams from a lower pass rate than the queue's gathers would pay)
       if (pass == 0 && !a.adaptive)
         eager = (uint32_t)tot * (SPEC == 3 && w.exact    ? (uint32_t)a.exact_eager_div
-                                 : SPEC == 3 && w.direct ? (uint32_t)a.direct_eager_div
+                                 : SPEC == 3 && w.direct ? (uint32_t)(a.occ_words > 0 ? a.occ_eager_div
+                                                                                      : a.direct_eager_div)
                                                          : 4u) > nv;
     }
     if (pass == 0) {

@@ -14,8 +14,9 @@ threshold, a lowered threshold (LJ_DIRECT_EAGER, with LJ_DIRECT=2: the
 in-place body even where the filter's selected keys predict few rows
 passing), the table's occupancy
 bitmap in LDS deciding the hits (LJ_DIRECT_OCC=2; a slot count that is not a
-multiple of 64 in the aliased case), the adaptive plan, and ragged and tiny
-fact tables.
+multiple of 64 in the aliased case) with the measure streamed or, below
+1/8 of the rows hitting (LJ_DIRECT_OCC_EAGER), gathered for the hits only,
+the adaptive plan, and ragged and tiny fact tables.
 """
 import os
 
@@ -77,7 +78,10 @@ def ctx_occ():
 
 @pytest.fixture(scope="module")
 def ctx_occ_low():
-    c = _context_with({"LJ_DIRECT": "2", "LJ_DIRECT_OCC": "2", "LJ_DIRECT_EAGER": "32"})
+    # ... and the hits' measures gathered, never streamed (streamed above
+    # 1/n of the rows hitting: n = 1 never streams)
+    c = _context_with({"LJ_DIRECT": "2", "LJ_DIRECT_OCC": "2", "LJ_DIRECT_EAGER": "32",
+                       "LJ_DIRECT_OCC_EAGER": "1"})
     yield c
     c.close()
 
@@ -89,6 +93,7 @@ CASES = [
     (1 << 16, (1 << 20) + 7, 3, 40, 1 << 15, L.HT_PHT2),    # table of n_dim/3 slots: keys alias
     (1 << 16, (1 << 20) + 1, 1, 5, 1 << 20, L.HT_PHT2),     # passes ~5 %: the queue
     (1 << 16, (1 << 20) + 3, 1, 12, 1 << 17, L.HT_PHT2),    # passes ~12 %: queue, or in place when lowered
+    (1 << 16, (1 << 20) + 9, 1, 2, 1 << 12, L.HT_PHT2),     # passes ~27 %, 2 % hit: measures gathered
     (1 << 12, 4099, 1, 70, 1 << 10, L.HT_PHT2),              # tiny, ragged
     (1 << 12, 100, 1, 70, 1 << 10, L.HT_PHT2),               # fewer rows than one tile
 ]
