@@ -1126,17 +1126,30 @@ static lj_status plan_probe(lj_ctx* c, const lj_star_desc* q, int want_product, 
   // body) whose occupancy bitmap fits in the LDS left: each workgroup builds
   // the bitmap from the table at its start (ht_len slots read from L2), and
   // the probe then decides hit or miss with an LDS read and gathers only the
-  // hits' payloads.  Repaid when each CU's share of the fact rows is at least
-  // the table's slot count (about 8 bytes read per slot at the start, against
-  // a slot gather saved per passing row).
+  // hits' payloads.  Repaid when the rows passing the filter, n * pass, are
+  // at least kOccRepay x CUs x slots (about 8 bytes read per slot by each
+  // workgroup, against a slot gather saved per passing row; measured: 2^25
+  // rows over 2^18 slots, a 0.5 ratio of rows to CUs x slots, gain at every
+  // pass rate from 35 % up).  The pass
+  // rate is estimated from the selected keys S as for direct1: known here
+  // for a resident dimension, checked by the probe on the build's counters
+  // (occ_min_sel) for one built with the query.
   if (a.direct1 && !partial && c->direct_occ > 0 &&
       (a.dim[0].bh.kind == BLOOM_MOD || a.dim[0].bh.kind == BLOOM_MOD_POW2)) {
-    const int64_t ns = a.dim[0].ht_len.d;
+    constexpr double kOccRepay = 0.125;
+    const ProbeDim& P = a.dim[0];
+    const int64_t ns = P.ht_len.d;
     const int64_t ow = (((ns + 63) / 64) * 2 + 3) & ~3ll;
-    const bool repaid = c->direct_occ >= 2 || (double)q->n_fact >= (double)c->n_cu * (double)ns;
-    if (repaid && lds_words + ow <= lds_budget_words) {
+    const double pass_min = kOccRepay * (double)c->n_cu * (double)ns / std::max<double>(1.0, (double)q->n_fact);
+    const double dom = (double)std::min<uint32_t>(P.bh.mod.d, P.ht_len.d);
+    const int64_t min_sel = c->direct_occ >= 2 ? 0 : pass_min < 1.0 ? (int64_t)std::ceil(-std::log1p(-pass_min) * dom)
+                                                                     : INT64_MAX;
+    bool want = min_sel < INT64_MAX;
+    if (want && q->resident[0]) want = p.dims[0]->n_selected >= min_sel;
+    if (want && lds_words + ow <= lds_budget_words) {
       a.occ_lds = lds_words;
       a.occ_words = (int32_t)ow;
+      a.occ_min_sel = q->resident[0] ? 0 : (int32_t)std::min<int64_t>(min_sel, INT32_MAX);
       lds_words += (int)ow;
     }
   }

@@ -146,6 +146,8 @@ struct ProbeArgs {
                            // its build selected at least this many keys (0: always)
   int32_t occ_eager_div;   // with the bitmap: the measure streams above 1 / occ_eager_div of the
                            // first tile's rows hitting (below, the hits' measures are gathered)
+  int32_t occ_min_sel;     // occ_words > 0, a dimension built with the query: the bitmap is built and
+                           // used when its build selected at least this many keys (0: always)
   // if set (a single query): the header and the first host_groups groups are
   // also written to this pinned host memory, so no copy follows the probe
   long long* host_hdr;

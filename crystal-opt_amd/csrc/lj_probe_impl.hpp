@@ -577,6 +577,7 @@ struct WaveCtx {
   int nd;
   bool exact;               // one dimension whose exact filter replaces the table probe
   bool direct;              // one dimension, one group: streamed survivors probe the table in place
+  bool occ;                 // direct, with the table's occupancy bitmap built in LDS
   long long acc_s;          // exact / direct: the lane's SUM(measure), row count and
   uint32_t acc_c;           // SUM(measure * payload), added in place
   long long acc_p;
@@ -1044,7 +1045,7 @@ __device__ __forceinline__ bool exact_in_place(const ProbeArgs& a, WaveCtx& w, W
 // slot 0 and drops it, so no branch separates the loads) -- and their
 // measures and products go to the lane's sums.  No queue: the same hits
 // (a nonzero slot), sums and statistics as the queued rows' complete_batch.
-// With the table's occupancy bitmap in LDS (occ_words), the bit decides the
+// With the table's occupancy bitmap in LDS (w.occ), the bit decides the
 // hit and only the hits' slots are gathered, for their payloads (products
 // over a PHT_2 table); the others read slot 0, one shared line.  Then the
 // measure need not stream: without it (have_meas false) the hits' measures
@@ -1057,7 +1058,7 @@ __device__ __forceinline__ int direct_in_place(const ProbeArgs& a, const uint32_
   int64_t sl[4];
   uint64_t raw[4];
   uint32_t hit = 0;
-  if (a.occ_words > 0) {
+  if (w.occ) {
     const uint32_t* occ = lds + a.occ_lds;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -1206,10 +1207,10 @@ __device__ __forceinline__ int tile_step(const ProbeArgs& a, const S0& s0, const
       st.v[ST_BLOOM_PASS] += cnt;  // every survivor passed the only filter
       continue;
     }
-    if (SPEC == 3 && w.direct && (have_eager || a.occ_words > 0)) {
+    if (SPEC == 3 && w.direct && (have_eager || w.occ)) {
       const int hits = direct_in_place(a, lds, w, st, kv, ev, m, row0, have_eager);
       st.v[ST_BLOOM_PASS] += cnt;
-      if (a.occ_words > 0) total += hits - cnt;  // with the bitmap, the measure streams by hits
+      if (w.occ) total += hits - cnt;  // with the bitmap, the measure streams by hits
       continue;
     }
     if (cnt > kDenseMin || have_eager) {
@@ -1521,7 +1522,15 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       }
     }
   }
-  if (SPEC == 3 && a.occ_words > 0) {
+  // the bitmap repays its build when enough rows pass the filter: for a
+  // dimension built with the query, judged from the keys its build selected
+  bool use_occ = SPEC == 3 && a.occ_words > 0;
+  if (use_occ && a.occ_min_sel > 0 && a.dim_counters[0]) {
+    unsigned long long sel = 0;
+    for (int l = 0; l < kCounterLanes; l++) sel += a.dim_counters[0][2 * l];
+    use_occ = sel >= (unsigned long long)a.occ_min_sel;
+  }
+  if (use_occ) {
     const ProbeDim& D = a.dim[0];
     if (D.ht_kind == HT_PHT1)
       occupancy_build<uint32_t, NW>(reinterpret_cast<const uint32_t*>(D.ht), D.ht_len.d, lds_filter + a.occ_lds, wave);
@@ -1570,6 +1579,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
     w.exact = o == 0;
   }
   w.direct = SPEC == 3 && !w.exact && a.n_dims == 1 && a.n_groups == 1 && a.direct1 != 0;
+  w.occ = w.direct && use_occ;
 
   const uint32_t total_waves = nblk * NW;
   const uint32_t gw = bid * NW + wave;
@@ -1644,7 +1654,7 @@ __device__ __forceinline__ void probe_body(const ProbeArgs& a, const uint32_t bi
       // streams from a lower pass rate than the queue's gathers would pay)
       if (pass == 0 && !a.adaptive)
         eager = (uint32_t)tot * (SPEC == 3 && w.exact    ? (uint32_t)a.exact_eager_div
-                                 : SPEC == 3 && w.direct ? (uint32_t)(a.occ_words > 0 ? a.occ_eager_div
+                                 : SPEC == 3 && w.direct ? (uint32_t)(w.occ ? a.occ_eager_div
                                                                                       : a.direct_eager_div)
                                                          : 4u) > nv;
     }

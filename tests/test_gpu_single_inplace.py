@@ -94,6 +94,7 @@ CASES = [
     (1 << 16, (1 << 20) + 1, 1, 5, 1 << 20, L.HT_PHT2),     # passes ~5 %: the queue
     (1 << 16, (1 << 20) + 3, 1, 12, 1 << 17, L.HT_PHT2),    # passes ~12 %: queue, or in place when lowered
     (1 << 16, (1 << 20) + 9, 1, 2, 1 << 12, L.HT_PHT2),     # passes ~27 %, 2 % hit: measures gathered
+    (1 << 12, (1 << 20) + 11, 1, 60, 1 << 11, L.HT_PHT2),   # small table: the default builds its bitmap
     (1 << 12, 4099, 1, 70, 1 << 10, L.HT_PHT2),              # tiny, ragged
     (1 << 12, 100, 1, 70, 1 << 10, L.HT_PHT2),               # fewer rows than one tile
 ]
@@ -138,15 +139,17 @@ def test_in_place_equals_queue_and_oracle(ctx, ctx_queue, ctx_low_threshold, ctx
             assert st["lines_in"][:2] == [ref["lines_fk"], ref["lines_measure"]], i
 
 
-@pytest.mark.parametrize("thr", [5, 50])
-def test_resident_dimension_matches_inline(ctx, thr):
-    """The in-place body is chosen from the filter's selected keys: on the
-    host for a resident dimension, on the device (the build's counters) for
-    one built with the query.  At 5 % selected (a 2^14-bit filter over
-    2^16 keys: about 18 % of the fact rows pass) neither chooses it, at 50 %
-    both do; either way the one-group star join equals the oracle's."""
-    n_dim, n_fact = 1 << 16, (1 << 20) + 5
-    _flush(f"resident vs inline one-dimension star join, threshold {thr}")
+@pytest.mark.parametrize("n_dim,thr", [(1 << 16, 5), (1 << 16, 50), (1 << 12, 50)])
+def test_resident_dimension_matches_inline(ctx, n_dim, thr):
+    """The in-place body and the table's occupancy bitmap are chosen from the
+    filter's selected keys: on the host for a resident dimension, on the
+    device (the build's counters) for one built with the query.  At 5 %
+    selected (a 2^14-bit filter over 2^16 keys: about 18 % of the fact rows
+    pass) neither chooses the body, at 50 % both do; over 2^12 keys both
+    also build the bitmap.  Either way the one-group star join equals the
+    oracle's."""
+    n_fact = (1 << 20) + 5
+    _flush(f"resident vs inline one-dimension star join, {n_dim} keys, threshold {thr}")
     keys, attr, fk, meas = W.single_join_host(SEED + 20 + thr, n_dim, n_fact, 99)
     dev = to_device({"keys": keys, "attr": attr, "fk": fk, "meas": meas})
     pred = W.single_join_pred(thr)
