@@ -168,3 +168,40 @@ def test_resident_dimension_matches_inline(ctx, n_dim, thr):
         assert r["stats"]["rows_in"][:2] == [n_fact, ref["n_match"]]
         assert r["stats"]["lines_in"][:2] == [ref["lines_fk"], ref["lines_measure"]]
         assert r["stats"]["rows_bloom_pass"] == ref["n_bloom_pass"]
+
+
+@pytest.mark.parametrize("per_kind", [False, True], ids=["one-launch", "launch-per-kind"])
+def test_batch_in_place_matches_alone(per_kind):
+    """One-dimension joins probed in place with the occupancy bitmap, as
+    instances of one batch (the batched probe kernels, one launch for
+    two stage-0 kinds or one per kind, eager and graph-replayed): each
+    result equals the same query run alone and the oracle's."""
+    _flush(f"batch of in-place single joins (per_kind={per_kind})")
+    c = _context_with({"LJ_DIRECT": "2", "LJ_DIRECT_OCC": "2"})
+    try:
+        # (n_dim, n_fact, threshold, filter bits): power-of-two and other bit counts
+        shapes = [(1 << 12, 300_007, 60, 1 << 11), (1 << 14, 500_000, 30, 3000),
+                  (1 << 13, 1 << 18, 5, 1 << 12), (1 << 12, 70_001, 90, 1500)]
+        descs, refs, keep = [], [], []
+        for i, (n_dim, n_fact, thr, bits) in enumerate(shapes):
+            keys, attr, fk, meas = W.single_join_host(SEED + 40 + i, n_dim, n_fact, 99)
+            dev = to_device({"keys": keys, "attr": attr, "fk": fk, "meas": meas})
+            pred = W.single_join_pred(thr)
+            d = L.dim_desc(dev["keys"], n_dim, n_dim, attr=dev["attr"], predicate=pred, bloom=("mod", bits))
+            descs.append(L.star_desc([dev["fk"]], dev["meas"], n_fact, [d], [0]))
+            keep.append(dev)
+            ref = O.single_join(O.DimSpec(keys, attr, n_dim, pred=list(pred), bloom=O.bloom_spec(O.BLOOM_MOD, bits, 1)),
+                                fk, meas, method=O.LIP, threads=8)
+            refs.append((ref["n_match"], ref["sum_measure"]))
+        alone = [c.star_join(q) for q in descs]
+        assert [(r["n_rows"], r["sum_total"]) for r in alone] == refs
+        for graph in (False, True):
+            b = c.batch(descs, use_graph=graph, launch_per_kind=per_kind)
+            for _ in range(2):
+                res = b.run()
+                assert [(r["n_rows"], r["sum_total"]) for r in res] == refs, graph
+                assert [r["stats"]["rows_in"][:2] for r in res] == [r["stats"]["rows_in"][:2] for r in alone]
+            b.close()
+        torch.cuda.synchronize()
+    finally:
+        c.close()
