@@ -205,3 +205,45 @@ def test_batch_in_place_matches_alone(per_kind):
         torch.cuda.synchronize()
     finally:
         c.close()
+
+
+def test_serve_in_place_bitmap():
+    """The serving loop (resident dimensions, continuous batching) over
+    one-dimension joins probed in place with the occupancy bitmap: every
+    request's count and sum equal the oracle's over its fact slice."""
+    _flush("serve in-place single joins")
+    c = _context_with({"LJ_DIRECT": "2", "LJ_DIRECT_OCC": "2"})
+    dims = []
+    try:
+        n_dim, n_pool = 1 << 14, 1 << 20
+        keys, attr, fk, meas = W.single_join_host(SEED + 60, n_dim, n_pool, 99)
+        dev = to_device({"keys": keys, "attr": attr, "fk": fk, "meas": meas})
+        specs = [(40, 1 << 12), (15, 3000)]
+        for thr, bits in specs:
+            d = L.dim_desc(dev["keys"], n_dim, n_dim, attr=dev["attr"], predicate=W.single_join_pred(thr),
+                           bloom=("mod", bits))
+            dims.append(c.build_dim(d))
+        rng = np.random.default_rng(SEED + 61)
+        reqs = []
+        for k in range(24):
+            rows = int(rng.integers(1000, 1 << 18))
+            off = int(rng.integers(0, (n_pool - rows) // 4 + 1)) * 4
+            reqs.append((k % len(specs), off, rows))
+        descs = [L.star_desc([dev["fk"][off:off + rows]], dev["meas"][off:off + rows], rows, None, [0],
+                             resident=[dims[t]]) for t, off, rows in reqs]
+        srv = c.serve(max_batch=8, max_inflight=2)
+        ids = [srv.submit(q) for q in descs]
+        out = [srv.poll(i, timeout_ms=60000) for i in ids]
+        srv.stop()
+        for (t, off, rows), o in zip(reqs, out):
+            thr, bits = specs[t]
+            ref = O.single_join(O.DimSpec(keys, attr, n_dim, pred=list(W.single_join_pred(thr)),
+                                          bloom=O.bloom_spec(O.BLOOM_MOD, bits, 1)),
+                                fk[off:off + rows], meas[off:off + rows], method=O.LIP, threads=8)
+            assert o is not None and o["status"] == L._abi.OK
+            assert (o["n_rows"], o["sum_total"]) == (ref["n_match"], ref["sum_measure"]), (t, off, rows)
+        torch.cuda.synchronize()
+    finally:
+        for d in dims:
+            d.release()
+        c.close()
